@@ -1,0 +1,560 @@
+// k-clique counting (3 <= k <= 5) on a degree-oriented DAG (replaces
+// src/CFSolver.cuh).
+//
+// The reference runs one 32-lane warp per oriented edge (v0, v1) and
+// materialises W1 = N+(v0) & N+(v1), W2 = W1 & N+(v2), ... in per-warp global
+// scratch (CFSolver.cuh:15-97).  Here every root vertex v0 instead builds the
+// DAG induced on its out-list S = N+(v0) once, as adjacency bitmaps over the
+// positions of S ("rows": bit j of row i says S[j] is in N+(S[i])), and then
+// counts (k-1)-cliques of that local DAG with AND + popcount:
+//     k=3: sum_i |row_i|
+//     k=4: sum_i sum_{j in row_i} |row_i & row_j|
+//     k=5: sum_i sum_{j in row_i} sum_{l in row_i & row_j} |row_i & row_j & row_l|
+// Row i equals the reference's W1 for the edge (v0, S[i]); row_i & row_j is its
+// W2.  Rows are built by streaming the lists N+(S[i]) through an LDS hash of S
+// (a flattened walk: all lanes read consecutive entries, whatever the list
+// lengths), or, for lists much longer than S, by binary-searching each S[j]
+// in N+(S[i]) and taking the 64-bit ballot as the row word.
+//
+// Three tiers by |S| (tiers.hpp): one wave per root with rows in registers
+// (|S| <= 64, one 64-bit word per row, lane i owns row i); one 256-thread
+// block per root with rows in LDS (|S| <= 512); one block per root with rows
+// in a per-block HBM region (larger |S|).
+//
+// MODE selects what is accumulated: kCount (the clique count), kBytes (the
+// algorithmic bytes of the reference's edge-rooted plan, see gpm.h), kWork
+// (per-root plan entries, the cost-model training target).
+#include <algorithm>
+#include <vector>
+
+#include "graph_dev.hpp"
+#include "tiers.hpp"
+
+using namespace gpm;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr uint32_t kWaveTable = 128;
+constexpr uint32_t kBlockCap = 512;
+enum { kCount = 0, kBytes = 1, kWork = 2 };
+
+__device__ __forceinline__ bool use_probe(uint32_t len, uint32_t d) {
+  return len > max(512u, 8u * d);
+}
+
+__device__ __forceinline__ uint32_t ctz64(uint64_t x) { return (uint32_t)__builtin_ctzll(x); }
+
+// ------------------------------------------------------------ wave tier
+template <int K, int MODE>
+__global__ __launch_bounds__(kBlock) void clique_wave_kernel(
+    const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
+    const uint32_t* __restrict__ roots, uint32_t n_roots, uint32_t chunk,
+    unsigned int* __restrict__ queue, unsigned long long* __restrict__ out,
+    unsigned long long* __restrict__ work) {
+  __shared__ uint32_t s_keys[kWavesPerBlock][kWaveTable];
+  __shared__ uint16_t s_vals[kWavesPerBlock][kWaveTable];
+  __shared__ unsigned long long s_rows[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_pre[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_beg[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_len[kWavesPerBlock][kWave];
+  const uint32_t lane = lane_id();
+  const uint32_t w = threadIdx.x / kWave;
+  uint32_t* keys = s_keys[w];
+  uint16_t* vals = s_vals[w];
+  unsigned long long* rows = s_rows[w];
+  uint32_t* pre = s_pre[w];
+  uint32_t* beg = s_beg[w];
+  uint32_t* lens = s_len[w];
+  constexpr uint32_t hmask = kWaveTable - 1;
+  uint64_t acc = 0;
+  while (true) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(queue, chunk);
+    base = wave_bcast(base, 0);
+    if (base >= n_roots) break;
+    const uint32_t stop = min(base + chunk, n_roots);
+    for (uint32_t r = base; r < stop; ++r) {
+      const uint32_t u = roots[r];
+      const uint32_t sb = off[u], d = off[u + 1] - sb;  // d <= 64 by tiering
+      const bool act = lane < d;
+      const uint32_t s = act ? adj[sb + lane] : kEmpty;
+      keys[lane] = kEmpty;
+      keys[lane + kWave] = kEmpty;
+      rows[lane] = 0ull;
+      __builtin_amdgcn_wave_barrier();
+      if (act) lds_hash_insert(keys, vals, hmask, s, (uint16_t)lane);
+      uint32_t b = 0, len = 0;
+      if (act) {
+        b = off[s];
+        len = off[s + 1] - b;
+      }
+      lens[lane] = len;
+      const bool probe = act && use_probe(len, d);
+      const uint32_t incl = wave_incl_scan(probe ? 0u : len);
+      pre[lane] = incl;
+      beg[lane] = b;
+      const uint32_t total = wave_bcast(incl, kWave - 1);
+      __builtin_amdgcn_wave_barrier();
+      for (uint32_t t = 0; t < total; t += kWave) {
+        const uint32_t f = t + lane;
+        if (f < total) {
+          const uint32_t j = upper_index(pre, kWave, f);
+          const uint32_t excl = j ? pre[j - 1] : 0u;
+          const uint32_t x = adj[beg[j] + (f - excl)];
+          const int idx = lds_hash_find(keys, vals, hmask, x);
+          if (idx >= 0) atomicOr(&rows[j], 1ull << idx);
+        }
+      }
+      uint64_t pm = wave_ballot(probe);
+      while (pm) {
+        const uint32_t i = ctz64(pm);
+        pm &= pm - 1;
+        const uint32_t bi = wave_bcast(b, i), li = wave_bcast(len, i);
+        const bool found = act && sorted_contains(adj + bi, li, s);
+        const uint64_t m = wave_ballot(found);
+        if (lane == 0) rows[i] = m;
+      }
+      __builtin_amdgcn_wave_barrier();
+      const uint64_t row = act ? rows[lane] : 0ull;
+      uint64_t c = 0;
+      if (MODE == kCount) {
+        if (K == 3) {
+          c = __popcll(row);
+        } else {
+          uint64_t bits = row;
+          while (bits) {
+            const uint32_t j = ctz64(bits);
+            bits &= bits - 1;
+            const uint64_t rj = row & rows[j];
+            if (K == 4) {
+              c += __popcll(rj);
+            } else {
+              uint64_t mm = rj;
+              while (mm) {
+                const uint32_t l = ctz64(mm);
+                mm &= mm - 1;
+                c += __popcll(rj & rows[l]);
+              }
+            }
+          }
+        }
+      } else if (act) {
+        // Edge (u, S[lane]) of the reference plan.
+        const uint64_t w1 = __popcll(row);
+        c = (uint64_t)d + len + (K >= 4 ? w1 : 0);
+        if (K >= 4) {
+          uint64_t bits = row;
+          while (bits) {
+            const uint32_t j = ctz64(bits);
+            bits &= bits - 1;
+            if (K == 4) {
+              c += w1 + lens[j];
+            } else {
+              const uint64_t w2m = row & rows[j];
+              const uint64_t w2 = __popcll(w2m);
+              c += w1 + lens[j] + w2;
+              uint64_t mm = w2m;
+              while (mm) {
+                const uint32_t l = ctz64(mm);
+                mm &= mm - 1;
+                c += w2 + lens[l];
+              }
+            }
+          }
+        }
+        c *= 4;  // bytes
+      }
+      if (MODE == kWork) {
+        const uint64_t tot = wave_sum_u64(c);
+        if (lane == 0) work[u] = tot / 4;
+      } else {
+        acc += c;
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  if (MODE != kWork) {
+    acc = wave_sum_u64(acc);
+    if (lane == 0 && acc) atomicAdd(out, (unsigned long long)acc);
+  }
+}
+
+// ------------------------------------------------------ block tiers
+
+__device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t* s_wsum) {
+  const uint32_t lane = lane_id(), w = threadIdx.x / kWave;
+  const uint32_t incl = wave_incl_scan(v);
+  if (lane == kWave - 1) s_wsum[w] = incl;
+  __syncthreads();
+  uint32_t add = 0;
+  for (uint32_t i = 0; i < w; ++i) add += s_wsum[i];
+  __syncthreads();
+  return incl + add;
+}
+
+// Per-root counting (or accounting) over W-word rows; thread-owned rows.
+template <int K, int MODE>
+__device__ __forceinline__ uint64_t count_rows(const unsigned long long* rows, uint32_t d,
+                                               uint32_t W, const uint32_t* lens) {
+  uint64_t c = 0;
+  for (uint32_t i = threadIdx.x; i < d; i += kBlock) {
+    const unsigned long long* ri = rows + (size_t)i * W;
+    if (MODE == kCount) {
+      if (K == 3) {
+        for (uint32_t w = 0; w < W; ++w) c += __popcll(ri[w]);
+        continue;
+      }
+      for (uint32_t w = 0; w < W; ++w) {
+        uint64_t bits = ri[w];
+        while (bits) {
+          const uint32_t j = w * 64 + ctz64(bits);
+          bits &= bits - 1;
+          const unsigned long long* rj = rows + (size_t)j * W;
+          if (K == 4) {
+            for (uint32_t x = 0; x < W; ++x) c += __popcll(ri[x] & rj[x]);
+          } else {
+            for (uint32_t x = 0; x < W; ++x) {
+              uint64_t mm = ri[x] & rj[x];
+              while (mm) {
+                const uint32_t l = x * 64 + ctz64(mm);
+                mm &= mm - 1;
+                const unsigned long long* rl = rows + (size_t)l * W;
+                for (uint32_t y = 0; y < W; ++y) c += __popcll(ri[y] & rj[y] & rl[y]);
+              }
+            }
+          }
+        }
+      }
+    } else {
+      uint64_t w1 = 0;
+      for (uint32_t w = 0; w < W; ++w) w1 += __popcll(ri[w]);
+      uint64_t e = (uint64_t)d + lens[i] + (K >= 4 ? w1 : 0);
+      if (K >= 4) {
+        for (uint32_t w = 0; w < W; ++w) {
+          uint64_t bits = ri[w];
+          while (bits) {
+            const uint32_t j = w * 64 + ctz64(bits);
+            bits &= bits - 1;
+            if (K == 4) {
+              e += w1 + lens[j];
+            } else {
+              const unsigned long long* rj = rows + (size_t)j * W;
+              uint64_t w2 = 0;
+              for (uint32_t x = 0; x < W; ++x) w2 += __popcll(ri[x] & rj[x]);
+              e += w1 + lens[j] + w2;
+              for (uint32_t x = 0; x < W; ++x) {
+                uint64_t mm = ri[x] & rj[x];
+                while (mm) {
+                  const uint32_t l = x * 64 + ctz64(mm);
+                  mm &= mm - 1;
+                  e += w2 + lens[l];
+                }
+              }
+            }
+          }
+        }
+      }
+      c += 4 * e;
+    }
+  }
+  return c;
+}
+
+// Block per root.  LDS_ROWS: rows, S, lens, begins and the hash table live in
+// dynamic LDS (|S| <= cap <= kBlockCap).  Otherwise rows live in this block's
+// HBM region `grows` (cap*W words) and membership in S is a binary search.
+template <int K, int MODE, bool LDS_ROWS>
+__global__ __launch_bounds__(kBlock) void clique_block_kernel(
+    const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
+    const uint32_t* __restrict__ roots, uint32_t n_roots, uint32_t cap,
+    uint32_t table, unsigned int* __restrict__ queue,
+    unsigned long long* __restrict__ grows, uint32_t* __restrict__ gmeta,
+    unsigned long long* __restrict__ out, unsigned long long* __restrict__ work) {
+  extern __shared__ unsigned long long s_dyn[];
+  __shared__ uint32_t s_pre[kBlock];
+  __shared__ uint32_t s_wsum[kWavesPerBlock];
+  __shared__ uint32_t s_root;
+  __shared__ int s_any_probe;
+  __shared__ unsigned long long s_red[kWavesPerBlock];
+  const uint32_t Wcap = (cap + 63) / 64;
+  unsigned long long* rows;
+  uint32_t *lens, *begs, *keys;
+  uint16_t* vals;
+  if (LDS_ROWS) {
+    rows = s_dyn;
+    lens = reinterpret_cast<uint32_t*>(rows + (size_t)cap * Wcap);
+    begs = lens + cap;
+    keys = begs + cap;
+    vals = reinterpret_cast<uint16_t*>(keys + table);
+  } else {
+    rows = grows + (size_t)blockIdx.x * cap * Wcap;
+    lens = gmeta + (size_t)blockIdx.x * 2 * cap;
+    begs = lens + cap;
+    keys = nullptr;
+    vals = nullptr;
+  }
+  const uint32_t hmask = table - 1;
+  uint64_t acc = 0;
+  while (true) {
+    if (threadIdx.x == 0) s_root = atomicAdd(queue, 1u);
+    __syncthreads();
+    const uint32_t r = s_root;
+    __syncthreads();
+    if (r >= n_roots) break;
+    const uint32_t u = roots[r];
+    const uint32_t sb = off[u], d = off[u + 1] - sb;
+    const uint32_t* S = adj + sb;
+    const uint32_t W = (d + 63) / 64;
+    if (threadIdx.x == 0) s_any_probe = 0;
+    for (uint32_t i = threadIdx.x; i < d; i += kBlock) {
+      const uint32_t v = S[i];
+      const uint32_t b = off[v];
+      const uint32_t len = off[v + 1] - b;
+      lens[i] = len;
+      begs[i] = b;
+    }
+    for (uint32_t i = threadIdx.x; i < d * W; i += kBlock) rows[i] = 0ull;
+    if (LDS_ROWS)
+      for (uint32_t i = threadIdx.x; i < table; i += kBlock) keys[i] = kEmpty;
+    __syncthreads();
+    // HBM rows: drop stale L1 lines of the previous root before reading.
+    if (!LDS_ROWS) __threadfence();
+    if (LDS_ROWS)
+      for (uint32_t i = threadIdx.x; i < d; i += kBlock)
+        lds_hash_insert(keys, vals, hmask, S[i], (uint16_t)i);
+    __syncthreads();
+    // Flattened walk over the lists that are streamed.
+    for (uint32_t c0 = 0; c0 < d; c0 += kBlock) {
+      const uint32_t i = c0 + threadIdx.x;
+      uint32_t wl = 0;
+      if (i < d) {
+        const uint32_t len = lens[i];
+        if (use_probe(len, d))
+          s_any_probe = 1;
+        else
+          wl = len;
+      }
+      const uint32_t incl = block_incl_scan(wl, s_wsum);
+      s_pre[threadIdx.x] = incl;
+      __syncthreads();
+      const uint32_t total = s_pre[kBlock - 1];
+      for (uint32_t f = threadIdx.x; f < total; f += kBlock) {
+        const uint32_t j = upper_index(s_pre, kBlock, f);
+        const uint32_t excl = j ? s_pre[j - 1] : 0u;
+        const uint32_t row_i = c0 + j;
+        const uint32_t x = adj[begs[row_i] + (f - excl)];
+        int idx;
+        if (LDS_ROWS) {
+          idx = lds_hash_find(keys, vals, hmask, x);
+        } else {
+          const uint32_t p = upper_index(S, d, x);
+          idx = (p > 0 && S[p - 1] == x) ? (int)(p - 1) : -1;
+        }
+        if (idx >= 0) atomicOr(&rows[(size_t)row_i * W + (idx >> 6)], 1ull << (idx & 63));
+      }
+      __syncthreads();
+    }
+    // Probed lists: the ballot of "S[t] in N+(S[i])" over 64 consecutive t
+    // is row word t/64.
+    if (s_any_probe) {
+      for (uint32_t i = 0; i < d; ++i) {
+        const uint32_t len = lens[i];
+        if (!use_probe(len, d)) continue;
+        const uint32_t* Ni = adj + begs[i];
+        for (uint32_t t0 = 0; t0 < d; t0 += kBlock) {
+          const uint32_t t = t0 + threadIdx.x;
+          const bool found = t < d && sorted_contains(Ni, len, S[t]);
+          const uint64_t m = wave_ballot(found);
+          const uint32_t word = t / 64;
+          if (lane_id() == 0 && word < W) rows[(size_t)i * W + word] = m;
+        }
+      }
+    }
+    __syncthreads();
+    if (!LDS_ROWS) __threadfence();
+    uint64_t c = count_rows<K, MODE>(rows, d, W, lens);
+    if (MODE == kWork) {
+      c = wave_sum_u64(c);
+      if (lane_id() == 0) s_red[threadIdx.x / kWave] = c;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < kWavesPerBlock; ++i) t += s_red[i];
+        work[u] = t / 4;
+      }
+    } else {
+      acc += c;
+    }
+    __syncthreads();
+  }
+  if (MODE != kWork) {
+    acc = wave_sum_u64(acc);
+    if (lane_id() == 0 && acc) atomicAdd(out, (unsigned long long)acc);
+  }
+}
+
+__global__ void zero_u32_kernel(unsigned int* p, int count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0u;
+}
+
+size_t block_lds_bytes(uint32_t cap, uint32_t table) {
+  const uint32_t W = (cap + 63) / 64;
+  return (size_t)cap * W * 8 + (size_t)cap * 8 + (size_t)table * 4 + (size_t)table * 2;
+}
+
+template <int K, int MODE>
+int launch_tiers(gpm_graph* g, const RootTiers* t, unsigned long long* out,
+                 unsigned long long* work) {
+  const int cus = device_cus(g);
+  unsigned int* q = t->queues;
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, g->stream, q, 4);
+  GPM_HIP(hipGetLastError());
+  if (t->n_big) {
+    const uint32_t cap = t->max_big_deg;
+    const uint32_t W = (cap + 63) / 64;
+    const unsigned blocks = (unsigned)std::min<uint64_t>(t->n_big, (uint64_t)cus);
+    const size_t row_bytes = (size_t)blocks * cap * W * 8;
+    const size_t meta_bytes = (size_t)blocks * 2 * cap * 4;
+    int st = ensure_scratch(g, row_bytes + meta_bytes + 256);
+    if (st != GPM_OK) return st;
+    unsigned long long* grows = static_cast<unsigned long long*>(g->scratch);
+    uint32_t* gmeta = reinterpret_cast<uint32_t*>(static_cast<char*>(g->scratch) + row_bytes);
+    hipLaunchKernelGGL((clique_block_kernel<K, MODE, false>), dim3(blocks), dim3(kBlock), 0,
+                       g->stream, g->off, g->adj, t->d_big, t->n_big, cap, 0u, q + 2, grows,
+                       gmeta, out, work);
+    GPM_HIP(hipGetLastError());
+  }
+  if (t->n_block) {
+    const uint32_t cap = t->max_block_deg;
+    uint32_t table = 1;
+    while (table < 2 * cap) table <<= 1;
+    const size_t lds = block_lds_bytes(cap, table);
+    const int per_cu = std::max<int>(1, (int)(160 * 1024 / (lds + 4096)));
+    const unsigned blocks =
+        (unsigned)std::min<uint64_t>(t->n_block, (uint64_t)cus * std::min(per_cu, 8));
+    hipLaunchKernelGGL((clique_block_kernel<K, MODE, true>), dim3(blocks), dim3(kBlock), lds,
+                       g->stream, g->off, g->adj, t->d_block, t->n_block, cap, table, q + 1,
+                       nullptr, nullptr, out, work);
+    GPM_HIP(hipGetLastError());
+  }
+  if (t->n_wave) {
+    const uint32_t chunk = t->wave_chunk;
+    const uint64_t waves_wanted = (t->n_wave + chunk - 1) / chunk;
+    const uint64_t waves = std::min<uint64_t>(waves_wanted, (uint64_t)cus * 32);
+    const unsigned blocks = (unsigned)std::max<uint64_t>(1, (waves + kWavesPerBlock - 1) / kWavesPerBlock);
+    hipLaunchKernelGGL((clique_wave_kernel<K, MODE>), dim3(blocks), dim3(kBlock), 0, g->stream,
+                       g->off, g->adj, t->d_wave, t->n_wave, chunk, q, out, work);
+    GPM_HIP(hipGetLastError());
+  }
+  return GPM_OK;
+}
+
+template <int MODE>
+int launch_k(gpm_graph* g, int k, const RootTiers* t, unsigned long long* out,
+             unsigned long long* work) {
+  switch (k) {
+    case 3: return launch_tiers<3, MODE>(g, t, out, work);
+    case 4: return launch_tiers<4, MODE>(g, t, out, work);
+    default: return launch_tiers<5, MODE>(g, t, out, work);
+  }
+}
+
+TierRequest request_from(const gpm_sched* s, uint32_t min_deg) {
+  TierRequest r;
+  r.min_deg = min_deg;
+  r.wave_cap = 64;
+  r.block_cap = kBlockCap;
+  if (s) {
+    r.root_begin = s->root_begin;
+    r.root_end = s->root_end;
+    r.policy = s->policy;
+    r.cost = s->root_cost;
+    r.chunk = s->chunk;
+    if (s->wave_cap && s->wave_cap < 64) r.wave_cap = s->wave_cap;
+  }
+  return r;
+}
+
+}  // namespace
+
+namespace gpm {
+// Heavy-root threshold for the learned policy: wave roots whose predicted
+// cost exceeds it are served by a whole block.
+float learned_heavy_cost(const float* cost, uint32_t n);
+}
+
+extern "C" {
+
+int gpm_clique(gpm_graph* g, int k, const gpm_sched* sched, uint64_t* out) {
+  clear_error();
+  GPM_CHECK_ARG(g && out, "null pointer");
+  GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "k-clique counting needs an ORIENTED graph");
+  GPM_CHECK_ARG(k >= 3 && k <= 5, "k must be 3, 4 or 5 (got %d)", k);
+  GPM_CHECK_ARG(!sched || sched->policy == GPM_SCHED_DEGREE || sched->policy == GPM_SCHED_LEARNED,
+                "unknown scheduling policy");
+  GPM_CHECK_ARG(!sched || sched->policy != GPM_SCHED_LEARNED || sched->root_cost,
+                "LEARNED policy needs root_cost");
+  TierRequest req = request_from(sched, (uint32_t)(k - 1));
+  if (req.policy == GPM_SCHED_LEARNED) req.heavy_cost = learned_heavy_cost(req.cost, g->n);
+  const RootTiers* t = nullptr;
+  int st = get_tiers(g, req, &t);
+  if (st != GPM_OK) return st;
+  if ((st = timing_begin(g)) != GPM_OK) return st;
+  if ((st = zero_counters(g, 1)) != GPM_OK) return st;
+  if ((st = launch_k<kCount>(g, k, t, g->counters, nullptr)) != GPM_OK) return st;
+  if ((st = timing_end(g)) != GPM_OK) return st;
+  unsigned long long h = 0;
+  GPM_HIP(hipMemcpyAsync(&h, g->counters, sizeof(h), hipMemcpyDeviceToHost, g->stream));
+  GPM_HIP(hipStreamSynchronize(g->stream));
+  *out = h;
+  return GPM_OK;
+}
+
+int gpm_clique_plan_bytes(gpm_graph* g, int k, uint64_t* bytes) {
+  clear_error();
+  GPM_CHECK_ARG(g && bytes, "null pointer");
+  GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "needs an ORIENTED graph");
+  GPM_CHECK_ARG(k >= 3 && k <= 5, "k must be 3, 4 or 5 (got %d)", k);
+  if (!g->plan_bytes[k]) {
+    TierRequest req = request_from(nullptr, 1);
+    const RootTiers* t = nullptr;
+    int st = get_tiers(g, req, &t);
+    if (st != GPM_OK) return st;
+    if ((st = zero_counters(g, 1)) != GPM_OK) return st;
+    if ((st = launch_k<kBytes>(g, k, t, g->counters, nullptr)) != GPM_OK) return st;
+    unsigned long long h = 0;
+    GPM_HIP(hipMemcpyAsync(&h, g->counters, sizeof(h), hipMemcpyDeviceToHost, g->stream));
+    GPM_HIP(hipStreamSynchronize(g->stream));
+    g->plan_bytes[k] = h + 4ull * (g->n + 1) + 8ull * g->m;
+  }
+  *bytes = g->plan_bytes[k];
+  return GPM_OK;
+}
+
+int gpm_clique_root_work(gpm_graph* g, int k, uint64_t* work) {
+  clear_error();
+  GPM_CHECK_ARG(g && work, "null pointer");
+  GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "needs an ORIENTED graph");
+  GPM_CHECK_ARG(k >= 3 && k <= 5, "k must be 3, 4 or 5 (got %d)", k);
+  TierRequest req = request_from(nullptr, 1);
+  const RootTiers* t = nullptr;
+  int st = get_tiers(g, req, &t);
+  if (st != GPM_OK) return st;
+  unsigned long long* d_work = nullptr;
+  GPM_HIP(hipMalloc(&d_work, (size_t)std::max(g->n, 1u) * 8));
+  GPM_HIP(hipMemsetAsync(d_work, 0, (size_t)std::max(g->n, 1u) * 8, g->stream));
+  st = launch_k<kWork>(g, k, t, g->counters, d_work);
+  if (st == GPM_OK) {
+    if (hipMemcpyAsync(work, d_work, (size_t)g->n * 8, hipMemcpyDeviceToHost, g->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(g->stream) != hipSuccess)
+      st = fail(GPM_EHIP, "work log copy failed");
+  }
+  hipFree(d_work);
+  return st;
+}
+
+}  // extern "C"

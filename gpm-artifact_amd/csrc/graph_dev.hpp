@@ -1,0 +1,60 @@
+// Device graph handle of libgpm.  Holds the CSR resident in HBM, the
+// graph's HIP stream and timing events, a host copy of the offsets (for
+// scheduling decisions) and reusable scratch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "device_common.hpp"
+#include "tiers.hpp"
+
+struct gpm_graph {
+  int device = 0;
+  uint32_t n = 0;
+  uint64_t m = 0;
+  uint32_t max_deg = 0;
+  uint32_t flags = 0;
+  uint32_t n_labels = 0;
+  // CSR (uint32 offsets and ids), resident on the device.
+  uint32_t* off = nullptr;
+  uint32_t* adj = nullptr;
+  // COO source of every adjacency entry, built on first use.
+  uint32_t* src = nullptr;
+  // Labels and the label-partitioned index: ladj holds every list sorted by
+  // (label, id); loff[v * (L + 1) + l] is where label l starts in v's list.
+  uint32_t* labels = nullptr;
+  uint32_t* ladj = nullptr;
+  uint32_t* loff = nullptr;
+  std::vector<uint32_t> h_off;
+  std::vector<uint32_t> h_labels;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  // Scratch: one growable device buffer plus a block of 64 counters.
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  unsigned long long* counters = nullptr;
+  // Root tiers built for earlier calls, keyed by the request that made them.
+  std::vector<std::pair<std::string, gpm::RootTiers>> tier_cache;
+  // Cached algorithmic-byte totals per k (0 = not computed yet).
+  uint64_t plan_bytes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+namespace gpm {
+
+// Grows g->scratch to at least `bytes`; returns status.
+int ensure_scratch(gpm_graph* g, size_t bytes);
+// Builds g->src (COO sources) if missing.
+int ensure_src(gpm_graph* g);
+// Zeroes the 64-entry counter block with a kernel (no memset node, so the
+// sequence stays valid inside a captured graph).
+int zero_counters(gpm_graph* g, int count);
+// Starts / stops event timing around the kernels of one mining call.
+int timing_begin(gpm_graph* g);
+int timing_end(gpm_graph* g);
+// Number of compute units of the handle's device.
+int device_cus(const gpm_graph* g);
+
+}  // namespace gpm

@@ -1,0 +1,121 @@
+// Root tier construction and caching (see tiers.hpp).
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+#include "graph_dev.hpp"
+
+namespace gpm {
+
+namespace {
+
+uint64_t checksum(const float* p, uint32_t n) {
+  uint64_t h = 1469598103934665603ull;
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(p);
+  for (uint32_t i = 0; i < n; ++i) h = (h ^ w[i]) * 1099511628211ull;
+  return h;
+}
+
+std::string key_of(const TierRequest& r, uint32_t n) {
+  char buf[256];
+  uint64_t cs = (r.policy == GPM_SCHED_LEARNED && r.cost) ? checksum(r.cost, n) : 0;
+  std::snprintf(buf, sizeof(buf), "%u/%u/%u/%u/%u/%d/%u/%g/%llx", r.min_deg, r.wave_cap,
+                r.block_cap, r.root_begin, r.root_end, r.policy, r.chunk, r.heavy_cost,
+                (unsigned long long)cs);
+  return buf;
+}
+
+int upload(const std::vector<uint32_t>& v, uint32_t** d) {
+  if (v.empty()) return GPM_OK;
+  if (hipMalloc(d, v.size() * 4) != hipSuccess) return fail(GPM_ENOMEM, "tier list");
+  if (hipMemcpy(*d, v.data(), v.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(GPM_EHIP, "tier list upload");
+  return GPM_OK;
+}
+
+}  // namespace
+
+int get_tiers(gpm_graph* g, const TierRequest& req, const RootTiers** out) {
+  const std::string key = key_of(req, g->n);
+  for (auto& kv : g->tier_cache)
+    if (kv.first == key) {
+      *out = &kv.second;
+      return GPM_OK;
+    }
+  const uint32_t rb = req.root_begin;
+  const uint32_t re = (req.root_end == 0 || req.root_end > g->n) ? g->n : req.root_end;
+  if (rb > re) return fail(GPM_EINVAL, "root range [%u, %u) is empty or reversed", rb, re);
+  std::vector<uint32_t> wave, block, big;
+  uint32_t maxb = 0, maxg = 0;
+  const bool learned = req.policy == GPM_SCHED_LEARNED && req.cost;
+  for (uint32_t v = rb; v < re; ++v) {
+    const uint32_t d = g->h_off[v + 1] - g->h_off[v];
+    if (d < req.min_deg) continue;
+    bool heavy = learned && req.heavy_cost > 0.f && req.cost[v] > req.heavy_cost;
+    if (d <= req.wave_cap && !heavy) {
+      wave.push_back(v);
+    } else if (d <= req.block_cap) {
+      block.push_back(v);
+      maxb = std::max(maxb, d);
+    } else {
+      big.push_back(v);
+      maxg = std::max(maxg, d);
+    }
+  }
+  if (learned) {
+    // Heaviest predicted roots first (longest-processing-time order), so the
+    // expensive subtrees start early and the tail of the launch is short.
+    auto by_cost = [&](uint32_t a, uint32_t b) {
+      return req.cost[a] > req.cost[b] || (req.cost[a] == req.cost[b] && a < b);
+    };
+    std::stable_sort(wave.begin(), wave.end(), by_cost);
+    std::stable_sort(block.begin(), block.end(), by_cost);
+    std::stable_sort(big.begin(), big.end(), by_cost);
+  }
+  RootTiers t;
+  t.n_wave = (uint32_t)wave.size();
+  t.n_block = (uint32_t)block.size();
+  t.n_big = (uint32_t)big.size();
+  t.max_block_deg = maxb;
+  t.max_big_deg = maxg;
+  t.wave_chunk = req.chunk ? req.chunk : 8;
+  int st;
+  if ((st = upload(wave, &t.d_wave)) != GPM_OK) return st;
+  if ((st = upload(block, &t.d_block)) != GPM_OK) return st;
+  if ((st = upload(big, &t.d_big)) != GPM_OK) return st;
+  if (hipMalloc(&t.queues, 4 * sizeof(unsigned int)) != hipSuccess)
+    return fail(GPM_ENOMEM, "work queues");
+  g->tier_cache.emplace_back(key, t);
+  *out = &g->tier_cache.back().second;
+  // Keep the cache small: learned requests with changing costs would grow it.
+  if (g->tier_cache.size() > 16) {
+    RootTiers& old = g->tier_cache.front().second;
+    hipFree(old.d_wave);
+    hipFree(old.d_block);
+    hipFree(old.d_big);
+    hipFree(old.queues);
+    g->tier_cache.erase(g->tier_cache.begin());
+    *out = &g->tier_cache.back().second;
+  }
+  return GPM_OK;
+}
+
+int get_tc_tiers(gpm_graph* g, uint32_t wave_cap, const RootTiers** out) {
+  TierRequest r;
+  r.min_deg = 2;
+  r.wave_cap = wave_cap;
+  r.block_cap = 0xFFFFFFFFu;
+  return get_tiers(g, r, out);
+}
+
+void free_tiers(gpm_graph* g) {
+  for (auto& kv : g->tier_cache) {
+    hipFree(kv.second.d_wave);
+    hipFree(kv.second.d_block);
+    hipFree(kv.second.d_big);
+    hipFree(kv.second.queues);
+  }
+  g->tier_cache.clear();
+}
+
+}  // namespace gpm

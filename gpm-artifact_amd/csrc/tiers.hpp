@@ -1,0 +1,47 @@
+// Root tiers: the host-side split of root vertices into size classes that
+// decides which kernel serves each root (one wave, one block with LDS rows,
+// one block with rows in HBM).  The plain degree scheduler orders each tier
+// by vertex id; the learned scheduler orders it by predicted cost (heaviest
+// first) and may move a root to a wider tier when its predicted cost says so.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+struct gpm_graph;
+
+namespace gpm {
+
+struct RootTiers {
+  uint32_t* d_wave = nullptr;   // roots served by one wave
+  uint32_t n_wave = 0;
+  uint32_t* d_block = nullptr;  // roots served by one block, rows in LDS
+  uint32_t n_block = 0;
+  uint32_t max_block_deg = 0;
+  uint32_t* d_big = nullptr;    // roots served by one block, rows in HBM
+  uint32_t n_big = 0;
+  uint32_t max_big_deg = 0;
+  unsigned int* queues = nullptr;  // 4 work-queue counters
+  uint32_t wave_chunk = 8;         // roots per wave fetch
+};
+
+struct TierRequest {
+  uint32_t min_deg = 1;       // roots below this out-degree do nothing
+  uint32_t wave_cap = 64;     // max out-degree of a wave root
+  uint32_t block_cap = 512;   // max out-degree of an LDS block root
+  uint32_t root_begin = 0, root_end = 0;
+  int policy = 0;             // gpm_sched_policy
+  const float* cost = nullptr;  // per-vertex predicted cost (learned)
+  uint32_t chunk = 0;         // 0 = automatic
+  float heavy_cost = 0.f;     // learned: wave roots above this go to blocks
+};
+
+// Returns cached tiers for (graph, request); builds and uploads on a miss.
+int get_tiers(gpm_graph* g, const TierRequest& req, const RootTiers** out);
+// Tiers for vertex-centric triangle counting (min_deg 2, wave_cap as given,
+// everything above goes to the block kernel).
+int get_tc_tiers(gpm_graph* g, uint32_t wave_cap, const RootTiers** out);
+void free_tiers(gpm_graph* g);
+
+}  // namespace gpm

@@ -1,0 +1,314 @@
+"""ctypes binding of libgpm (include/gpm.h) for tests, bench and scripts.
+
+The engine is the HIP library; this module only converts numpy arrays to
+pointers and status codes to exceptions.  There is no fallback: if
+lib/libgpm.so is missing, every call raises GpmError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libgpm.so")
+
+# gpm.h constants
+GPM_OK = 0
+CSR_SYMMETRIZE, CSR_DEDUP, CSR_DROP_SELF, CSR_SORT = 0x1, 0x2, 0x4, 0x8
+CSR_CLEAN = CSR_SYMMETRIZE | CSR_DEDUP | CSR_DROP_SELF | CSR_SORT
+GRAPH_ORIENTED, GRAPH_SYMMETRIC = 0x1, 0x2
+SCHED_DEGREE, SCHED_LEARNED = 0, 1
+TC_EDGE, TC_VERTEX = 0, 1
+FORM_G2MINER, FORM_PATTERN = 0, 1
+PATTERNS = {"P1": 1, "P2": 2, "P3": 3, "P4": 4, "P5": 5, "P6": 6, "P7": 7,
+            "P8": 8, "Motif4": 9}
+
+_u32p = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+_f32p = C.POINTER(C.c_float)
+
+
+class GpmError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"gpm status {status}: {msg}")
+        self.status = status
+
+
+class Sched(C.Structure):
+    _fields_ = [("policy", C.c_int), ("chunk", C.c_uint32),
+                ("root_begin", C.c_uint32), ("root_end", C.c_uint32),
+                ("root_cost", _f32p), ("wave_cap", C.c_uint32),
+                ("reserved", C.c_uint32 * 3)]
+
+
+class Query(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("n_edges", C.c_uint32),
+                ("edges", _u32p), ("labels", _u32p)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("autotune", C.c_int), ("policy", C.c_int),
+                ("chunk", C.c_uint32), ("use_graph", C.c_int),
+                ("reserved", C.c_uint32 * 4)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads libgpm.so once; raises GpmError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GpmError(-1, f"{LIB_PATH} not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.gpm_last_error.restype = C.c_char_p
+        L.gpm_free.argtypes = [C.c_void_p]
+        L.gpm_free.restype = None
+        _lib = L
+    return _lib
+
+
+def _check(st: int) -> None:
+    if st != GPM_OK:
+        raise GpmError(st, lib().gpm_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray, ctype=C.c_uint32):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _take(p, count: int, dtype=np.uint32) -> np.ndarray:
+    """Copies a library-owned array into numpy and frees it."""
+    if count == 0:
+        out = np.zeros(0, dtype=dtype)
+    else:
+        buf = (C.c_uint32 * count).from_address(C.cast(p, C.c_void_p).value)
+        out = np.frombuffer(buf, dtype=dtype, count=count).copy()
+    lib().gpm_free(C.cast(p, C.c_void_p))
+    return out
+
+
+@dataclass
+class HostCSR:
+    offsets: np.ndarray  # uint32, n+1
+    adj: np.ndarray      # uint32, m
+
+    @property
+    def n(self) -> int:
+        return len(self.offsets) - 1
+
+    @property
+    def m(self) -> int:
+        return len(self.adj)
+
+    def degrees(self) -> np.ndarray:
+        return np.diff(self.offsets.astype(np.int64))
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+def csr_from_edges(n: int, src, dst, flags: int = CSR_CLEAN) -> HostCSR:
+    src, dst = _u32(src), _u32(dst)
+    assert len(src) == len(dst)
+    po, pa, m = _u32p(), _u32p(), C.c_uint64()
+    _check(lib().gpm_csr_from_edges(C.c_uint32(n), _ptr(src), _ptr(dst),
+                                    C.c_uint64(len(src)), C.c_uint32(flags),
+                                    C.byref(po), C.byref(pa), C.byref(m)))
+    return HostCSR(_take(po, n + 1), _take(pa, m.value))
+
+
+def orient(g: HostCSR) -> HostCSR:
+    po, pa, m = _u32p(), _u32p(), C.c_uint64()
+    _check(lib().gpm_csr_orient(C.c_uint32(g.n), _ptr(g.offsets), _ptr(g.adj),
+                                C.byref(po), C.byref(pa), C.byref(m)))
+    return HostCSR(_take(po, g.n + 1), _take(pa, m.value))
+
+
+def sort_by_degree(g: HostCSR):
+    po, pa, pp = _u32p(), _u32p(), _u32p()
+    _check(lib().gpm_csr_sort_by_degree(C.c_uint32(g.n), _ptr(g.offsets),
+                                        _ptr(g.adj), C.byref(po), C.byref(pa),
+                                        C.byref(pp)))
+    return HostCSR(_take(po, g.n + 1), _take(pa, g.m)), _take(pp, g.n)
+
+
+def load_mtx(path: str):
+    n, ps, pd, ne = C.c_uint32(), _u32p(), _u32p(), C.c_uint64()
+    _check(lib().gpm_load_mtx(path.encode(), C.byref(n), C.byref(ps),
+                              C.byref(pd), C.byref(ne)))
+    return n.value, _take(ps, ne.value), _take(pd, ne.value)
+
+
+def gen_er(n: int, m: int, seed: int):
+    ps, pd, ne = _u32p(), _u32p(), C.c_uint64()
+    _check(lib().gpm_gen_er(C.c_uint32(n), C.c_uint64(m), C.c_uint64(seed),
+                            C.byref(ps), C.byref(pd), C.byref(ne)))
+    return _take(ps, ne.value), _take(pd, ne.value)
+
+
+def gen_rmat(scale: int, edge_factor: int, seed: int, a=0.57, b=0.19, c=0.19):
+    ps, pd, ne = _u32p(), _u32p(), C.c_uint64()
+    _check(lib().gpm_gen_rmat(C.c_int(scale), C.c_int(edge_factor),
+                              C.c_double(a), C.c_double(b), C.c_double(c),
+                              C.c_uint64(seed), C.byref(ps), C.byref(pd),
+                              C.byref(ne)))
+    return _take(ps, ne.value), _take(pd, ne.value)
+
+
+def gen_labels(n: int, n_labels: int, seed: int) -> np.ndarray:
+    pl = _u32p()
+    _check(lib().gpm_gen_labels(C.c_uint32(n), C.c_uint32(n_labels),
+                                C.c_uint64(seed), C.byref(pl)))
+    return _take(pl, n)
+
+
+def er_graph(n: int, m: int, seed: int) -> HostCSR:
+    """Symmetric, deduplicated, sorted G(n, M) CSR."""
+    s, d = gen_er(n, m, seed)
+    return csr_from_edges(n, s, d, CSR_CLEAN)
+
+
+def rmat_graph(scale: int, edge_factor: int, seed: int, a=0.57, b=0.19,
+               c=0.19) -> HostCSR:
+    """Symmetrised, deduplicated, self-loop-free, sorted R-MAT CSR."""
+    s, d = gen_rmat(scale, edge_factor, seed, a, b, c)
+    return csr_from_edges(1 << scale, s, d, CSR_CLEAN)
+
+
+class DeviceGraph:
+    """A CSR resident in HBM (gpm_graph handle)."""
+
+    def __init__(self, g: HostCSR, oriented: bool, labels=None,
+                 n_labels: int = 0, _handle=None):
+        self._h = C.c_void_p()
+        self.oriented = oriented
+        if _handle is not None:
+            self._h = _handle
+            return
+        flags = GRAPH_ORIENTED if oriented else GRAPH_SYMMETRIC
+        lab_ptr = None
+        if labels is not None:
+            self._labels = _u32(labels)
+            lab_ptr = _ptr(self._labels)
+            n_labels = n_labels or int(self._labels.max()) + 1
+        _check(lib().gpm_graph_create(_ptr(_u32(g.offsets)), C.c_uint32(g.n),
+                                      _ptr(_u32(g.adj)), C.c_uint64(g.m),
+                                      lab_ptr, C.c_uint32(n_labels),
+                                      C.c_uint32(flags), C.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().gpm_graph_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        n, m, d = C.c_uint32(), C.c_uint64(), C.c_uint32()
+        _check(lib().gpm_graph_info(self._h, C.byref(n), C.byref(m), C.byref(d)))
+        return n.value, m.value, d.value
+
+    def orient(self) -> "DeviceGraph":
+        h = C.c_void_p()
+        _check(lib().gpm_graph_orient(self._h, C.byref(h)))
+        return DeviceGraph(None, True, _handle=h)
+
+    def last_kernel_ms(self) -> float:
+        ms = C.c_double()
+        _check(lib().gpm_last_kernel_ms(self._h, C.byref(ms)))
+        return ms.value
+
+    def tc(self, variant: int = TC_VERTEX) -> int:
+        out = C.c_uint64()
+        _check(lib().gpm_tc(self._h, C.c_int(variant), C.byref(out)))
+        return out.value
+
+    def clique(self, k: int, policy: int = SCHED_DEGREE, chunk: int = 0,
+               root_begin: int = 0, root_end: int = 0,
+               cost: Optional[np.ndarray] = None, wave_cap: int = 0) -> int:
+        s = Sched()
+        s.policy, s.chunk = policy, chunk
+        s.root_begin, s.root_end, s.wave_cap = root_begin, root_end, wave_cap
+        if cost is not None:
+            self._cost = np.ascontiguousarray(cost, dtype=np.float32)
+            s.root_cost = _ptr(self._cost, C.c_float)
+        out = C.c_uint64()
+        _check(lib().gpm_clique(self._h, C.c_int(k), C.byref(s), C.byref(out)))
+        return out.value
+
+    def clique_plan_bytes(self, k: int) -> int:
+        out = C.c_uint64()
+        _check(lib().gpm_clique_plan_bytes(self._h, C.c_int(k), C.byref(out)))
+        return out.value
+
+    def clique_root_work(self, k: int) -> np.ndarray:
+        n = self.info()[0]
+        w = np.zeros(n, dtype=np.uint64)
+        _check(lib().gpm_clique_root_work(self._h, C.c_int(k), _ptr(w, C.c_uint64)))
+        return w
+
+    def pattern(self, name: str, formulation: int = FORM_G2MINER):
+        out = (C.c_uint64 * 6)()
+        scalar = C.c_uint64()
+        _check(lib().gpm_pattern(self._h, C.c_int(PATTERNS[name]),
+                                 C.c_int(formulation), out, C.byref(scalar)))
+        return [int(x) for x in out], scalar.value
+
+    def match_batch(self, queries: Sequence, autotune: int = 1,
+                    policy: int = SCHED_DEGREE, chunk: int = 0,
+                    use_graph: int = 1) -> np.ndarray:
+        """queries: sequence of (n, edges [(u, v), ...], labels [..])."""
+        arr = (Query * len(queries))()
+        keep = []
+        for i, (qn, edges, labels) in enumerate(queries):
+            e = _u32(np.asarray(edges, dtype=np.uint32).reshape(-1))
+            l = _u32(labels)
+            keep += [e, l]
+            arr[i].n, arr[i].n_edges = qn, len(e) // 2
+            arr[i].edges, arr[i].labels = _ptr(e), _ptr(l)
+        t = Tuning()
+        t.autotune, t.policy, t.chunk, t.use_graph = autotune, policy, chunk, use_graph
+        out = np.zeros(len(queries), dtype=np.uint64)
+        _check(lib().gpm_match_batch(self._h, arr, C.c_int(len(queries)),
+                                     C.byref(t), _ptr(out, C.c_uint64)))
+        return out
+
+
+def vertex_features(sym: DeviceGraph, dag: Optional[DeviceGraph]) -> np.ndarray:
+    n = sym.info()[0]
+    f = np.zeros(4 * n, dtype=np.float32)
+    _check(lib().gpm_vertex_features(sym.handle, dag.handle if dag else None,
+                                     _ptr(f, C.c_float)))
+    return f.reshape(n, 4)
+
+
+def mlp_predict(feat: np.ndarray, mean, std, w1, b1, w2, b2, w3, b3) -> np.ndarray:
+    feat = np.ascontiguousarray(feat, dtype=np.float32)
+    arrs = [np.ascontiguousarray(x, dtype=np.float32)
+            for x in (mean, std, w1, b1, w2, b2, w3, b3)]
+    out = np.zeros(len(feat), dtype=np.float32)
+    _check(lib().gpm_mlp_predict(_ptr(feat, C.c_float), C.c_uint32(len(feat)),
+                                 *[_ptr(a, C.c_float) for a in arrs],
+                                 _ptr(out, C.c_float)))
+    return out

@@ -1,0 +1,264 @@
+/*
+ * gpm.h -- C ABI of the MI355X (gfx950, wave64) graph-pattern-mining engine.
+ *
+ * Every entry point takes plain pointers and sizes and returns an int status
+ * (GPM_OK == 0).  On failure the thread-local message behind gpm_last_error()
+ * says what went wrong; the C++ solver shims in gpm-artifact_amd/include turn a
+ * non-zero status into the reference's abort-with-log behaviour.
+ *
+ * Conventions
+ *   - vertex ids and CSR offsets are uint32 (the reference uses int VID for
+ *     both, src/graph/graph.h:422-423); graphs with >= 2^32 adjacency entries
+ *     are rejected with GPM_EINVAL.
+ *   - adjacency lists are sorted ascending and hold no duplicates or self loops
+ *     unless a function says otherwise.
+ *   - counts are uint64 (AccType = unsigned long long, src/utils/utils.h:25).
+ *   - host arrays passed in are copied; arrays handed out (uint32_t** outs) are
+ *     malloc'ed by the library and released with gpm_free().
+ *   - device-side calls are synchronous (like the reference solvers, which
+ *     end with a device synchronise + D2H of the counters) and run on the
+ *     graph's own HIP stream; one graph handle must not be used by two host
+ *     threads at once.
+ */
+#ifndef GPM_H_
+#define GPM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum gpm_status {
+  GPM_OK = 0,
+  GPM_EINVAL = 1,      /* bad argument (null pointer, size, unknown id)      */
+  GPM_EHIP = 2,        /* a HIP runtime call failed                          */
+  GPM_ENOMEM = 3,      /* host or device allocation failed                   */
+  GPM_EUNSUPPORTED = 4,/* valid request the engine does not implement        */
+  GPM_EIO = 5          /* file could not be opened / parsed                  */
+};
+
+/* Message for the last failing call on this host thread ("" if none). */
+const char* gpm_last_error(void);
+/* ABI version, bumped on any signature change. */
+int gpm_abi_version(void);
+/* Release an array the library handed out. */
+void gpm_free(void* p);
+
+/* ------------------------------------------------ host graph preparation */
+
+/* Flags for gpm_csr_from_edges. */
+#define GPM_CSR_SYMMETRIZE 0x1u /* add (v,u) for every (u,v)                 */
+#define GPM_CSR_DEDUP 0x2u      /* drop repeated (u,v)                         */
+#define GPM_CSR_DROP_SELF 0x4u  /* drop (u,u)                                  */
+#define GPM_CSR_SORT 0x8u       /* sort every adjacency list ascending         */
+
+/*
+ * Edge list -> CSR.  With flags == 0 this is Graph::Init's CSR build: lists
+ * keep input order (src/graph/graph.h:271-287).  The configurations' graphs
+ * use SYMMETRIZE|DEDUP|DROP_SELF|SORT.
+ */
+int gpm_csr_from_edges(uint32_t n, const uint32_t* src, const uint32_t* dst,
+                       uint64_t n_edges, uint32_t flags, uint32_t** offsets,
+                       uint32_t** adj, uint64_t* m);
+
+/*
+ * Degree orientation, replacing Graph::orientation (src/graph/graph.h:116-204):
+ * keep u->v iff deg(v) > deg(u) or (deg(v) == deg(u) and v > u), where deg is
+ * the list length in the input CSR (graph.h:128-129).  List order is kept.
+ */
+int gpm_csr_orient(uint32_t n, const uint32_t* offsets, const uint32_t* adj,
+                   uint32_t** out_offsets, uint32_t** out_adj, uint64_t* out_m);
+
+/*
+ * Degree relabel, replacing Graph::SortCSRGraph (src/graph/graph.h:206-252):
+ * stable sort of vertices by list length, descending; new_id[old] is written
+ * to *perm (n entries) and every list is renamed and sorted ascending.
+ */
+int gpm_csr_sort_by_degree(uint32_t n, const uint32_t* offsets,
+                           const uint32_t* adj, uint32_t** out_offsets,
+                           uint32_t** out_adj, uint32_t** perm);
+
+/*
+ * Matrix-market edge reader with the semantics of Loader::_load_mtx
+ * (src/graph/io.h:84-139): comment lines are skipped, the "n n m" header is
+ * read and ignored, self loops are dropped, ids are rebased by the minimum id
+ * seen and n = max - min + 1.  No symmetrisation, dedup or sort.
+ */
+int gpm_load_mtx(const char* path, uint32_t* n, uint32_t** src, uint32_t** dst,
+                 uint64_t* n_edges);
+
+/*
+ * Seeded Erdos-Renyi G(n, M): M distinct undirected edges, no self loops,
+ * emitted once each as (min, max).  No counterpart in the reference (which
+ * only reads files); used for the triangle-counting workload.
+ */
+int gpm_gen_er(uint32_t n, uint64_t m_undirected, uint64_t seed,
+               uint32_t** src, uint32_t** dst, uint64_t* n_edges);
+
+/*
+ * Seeded R-MAT: edge_factor * 2^scale samples with quadrant probabilities
+ * (a, b, c, 1-a-b-c), no noise and no permutation.  Raw samples (self loops
+ * and duplicates included); build the CSR with gpm_csr_from_edges.
+ */
+int gpm_gen_rmat(int scale, int edge_factor, double a, double b, double c,
+                 uint64_t seed, uint32_t** src, uint32_t** dst,
+                 uint64_t* n_edges);
+
+/* Seeded uniform labels in [0, n_labels).  Replaces the "randomK" branch of
+ * Loader::LoadVLabel (src/graph/io.h:43-49), which used unseeded rand(). */
+int gpm_gen_labels(uint32_t n, uint32_t n_labels, uint64_t seed,
+                   uint32_t** labels);
+
+/* --------------------------------------------------------- device graph */
+
+typedef struct gpm_graph gpm_graph;
+
+#define GPM_GRAPH_ORIENTED 0x1u /* CSR is a DAG (TC, k-clique)               */
+#define GPM_GRAPH_SYMMETRIC 0x2u /* CSR is undirected (patterns, matching)   */
+
+/*
+ * Upload a CSR (and optional labels) to the current HIP device; the handle
+ * owns the device copy until gpm_graph_destroy.  Replaces Graph::Init's H2D
+ * copies and Graph::DeviceObject (src/graph/graph.h:311-366).  labels may be
+ * NULL; when given, a label-partitioned adjacency index is also built for
+ * the matcher.
+ */
+int gpm_graph_create(const uint32_t* offsets, uint32_t n, const uint32_t* adj,
+                     uint64_t m, const uint32_t* labels, uint32_t n_labels,
+                     uint32_t flags, gpm_graph** out);
+int gpm_graph_destroy(gpm_graph* g);
+/* n, m, max list length. */
+int gpm_graph_info(const gpm_graph* g, uint32_t* n, uint64_t* m,
+                   uint32_t* max_deg);
+/*
+ * Device-side orientation of a SYMMETRIC graph into a new ORIENTED handle,
+ * same rule as gpm_csr_orient (src/graph/graph.h:116-204), computed on the GPU.
+ */
+int gpm_graph_orient(gpm_graph* sym, gpm_graph** dag);
+
+/* Event-timed duration (ms) of the kernels of the last mining call on g. */
+int gpm_last_kernel_ms(const gpm_graph* g, double* ms);
+
+/* ------------------------------------------------------------ scheduling */
+
+enum gpm_sched_policy {
+  GPM_SCHED_DEGREE = 0,  /* plain degree-based scheduler (reference style)  */
+  GPM_SCHED_LEARNED = 1, /* cost-model order + chunking from per-root costs */
+};
+
+typedef struct gpm_sched {
+  int policy;              /* gpm_sched_policy                               */
+  uint32_t chunk;          /* roots per work-queue fetch, 0 = automatic       */
+  uint32_t root_begin;     /* root range [root_begin, root_end) (vertex ids); */
+  uint32_t root_end;       /*   root_end == 0 means "all vertices"            */
+  const float* root_cost;  /* LEARNED: predicted cost per vertex (n floats)   */
+  uint32_t wave_cap;       /* local-set size served by one wave, 0 = 64       */
+  uint32_t reserved[3];
+} gpm_sched;
+
+/* ------------------------------------------------------------- mining */
+
+enum gpm_tc_variant {
+  GPM_TC_EDGE = 0,   /* e_centric: edge-parallel binary-search intersection */
+  GPM_TC_VERTEX = 1, /* v_centric: vertex-parallel LDS hash table           */
+};
+
+/*
+ * Triangle count of an ORIENTED graph, replacing TCSolver
+ * (src/TCSolver.cuh:10-121; e_centric :27-59, v_centric :64-113).
+ */
+int gpm_tc(gpm_graph* g, int variant, uint64_t* out);
+
+/*
+ * k-clique count (3 <= k <= 5) of an ORIENTED graph, replacing CFSolver
+ * (src/CFSolver.cuh:99-157, which supports k = 4, 5).  sched may be NULL.
+ */
+int gpm_clique(gpm_graph* g, int k, const gpm_sched* sched, uint64_t* out);
+
+/*
+ * Per-root work log for the cost model: work[v] = number of adjacency
+ * entries the plain k-clique plan scans for root vertex v (n entries).
+ */
+int gpm_clique_root_work(gpm_graph* g, int k, uint64_t* work);
+
+/*
+ * Algorithmic bytes of the canonical edge-rooted k-clique plan
+ * (CFSolver.cuh:15-97: materialise W1 = N+(v0) & N+(v1), ...):
+ * 4(n+1) + 8|E+| + sum over set operations of 4(|A|+|B|) (+ 4|out| when the
+ * result is materialised).  k == 3 gives the triangle-counting figure.
+ */
+int gpm_clique_plan_bytes(gpm_graph* g, int k, uint64_t* bytes);
+
+enum gpm_pattern_id {
+  GPM_P1 = 1, GPM_P2, GPM_P3, GPM_P4, GPM_P5, GPM_P6, GPM_P7, GPM_P8,
+  GPM_MOTIF4 = 9
+};
+enum gpm_formulation {
+  GPM_FORM_G2MINER = 0, /* direct enumeration (G2MinerSolver)              */
+  GPM_FORM_PATTERN = 1, /* frequency x count - correction (PatternSolver)   */
+};
+
+/*
+ * Fixed-pattern count on a SYMMETRIC graph, replacing G2MinerSolver
+ * (src/G2MinerSolver.cuh:18-175) and PatternSolver (src/PatternSolver.cuh:
+ * 17-264).  out[0..5]: for P1..P8 out[0] is the pattern count; for MOTIF4
+ * out = induced {3-star, 4-path, tailed-triangle, 4-cycle, diamond,
+ * 4-clique}.  *scalar receives what the matching reference solver returns
+ * (G2Miner: counters[0]; PatternSolver: h[1] or h[1]-h[2], modulo 2^64).
+ */
+int gpm_pattern(gpm_graph* g, int pattern_id, int formulation,
+                uint64_t out[6], uint64_t* scalar);
+
+/* Query graph for the labeled matcher (<= 16 vertices, connected). */
+typedef struct gpm_query {
+  uint32_t n;               /* vertices                                     */
+  uint32_t n_edges;         /* undirected edges                              */
+  const uint32_t* edges;    /* 2*n_edges endpoints (u0,v0,u1,v1,...)         */
+  const uint32_t* labels;   /* n labels                                      */
+} gpm_query;
+
+typedef struct gpm_tuning {
+  int autotune;             /* 1: project autotuner picks per-query knobs    */
+  int policy;               /* gpm_sched_policy for the matching order      */
+  uint32_t chunk;           /* candidate roots per task, 0 = automatic       */
+  int use_graph;            /* 1: capture the batch into one hipGraph        */
+  uint32_t reserved[4];
+} gpm_tuning;
+
+/*
+ * Labeled subgraph matching: number of injective, label-preserving,
+ * edge-preserving maps of each query into a SYMMETRIC labeled data graph
+ * (non-induced; automorphisms counted) -- the count the reference's CPU CECI
+ * matcher prints as "#Embeddings" (sub_ceci/myceci.cpp:540-668, 1115).
+ * All queries run in one persistent launch; per_query[i] is query i's count.
+ */
+int gpm_match_batch(gpm_graph* g, const gpm_query* queries, int n_queries,
+                    const gpm_tuning* tuning, uint64_t* per_query);
+
+/* ------------------------------------------------------- cost model (MLP) */
+
+/*
+ * Per-vertex features for the work estimator, computed on the device:
+ * feat[4*v + {0,1,2,3}] = {deg(v), out-degree in dag, 2-hop size bound
+ * sum_{u in N(v)} deg(u), label}.  dag may be NULL (out-degree 0).
+ */
+int gpm_vertex_features(gpm_graph* sym, gpm_graph* dag, float* feat);
+
+/*
+ * 3-layer MLP (4 -> 64 -> 64 -> 1, ReLU) on BF16 matrix cores with FP32
+ * accumulation: out[v] = W3 relu(W2 relu(W1 x_v + b1) + b2) + b3, x_v being
+ * the 4 features of v after log1p and the given affine normalisation.
+ * Weights are FP32 row-major (W1 64x4, W2 64x64, W3 1x64) and rounded to BF16.
+ */
+int gpm_mlp_predict(const float* feat, uint32_t n, const float* norm_mean,
+                    const float* norm_std, const float* w1, const float* b1,
+                    const float* w2, const float* b2, const float* w3,
+                    const float* b3, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPM_H_ */

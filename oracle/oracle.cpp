@@ -1,0 +1,633 @@
+/*
+ * oracle.cpp -- CPU restatement of the reference's mining algorithms.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it, and only
+ * to compare against or to time beside the HIP engine.  The engine never
+ * calls it.
+ *
+ * Every count is the mathematical count the reference computes; where a
+ * reference kernel uses an int32 per-thread accumulator (a possible overflow)
+ * this restatement uses uint64 throughout.
+ *
+ * Pinning (what checks each function beyond retyping it):
+ *   orc_tc            closed forms (K_n, K_ab, wheels), networkx triangles,
+ *                     orientation/relabel invariance          tests/test_oracle.py
+ *   orc_clique        closed forms (K_n, Turan graphs), networkx clique
+ *                     enumeration                             tests/test_oracle.py
+ *   orc_plan_bytes    brute-force recount of the plan's set sizes in Python
+ *   orc_motif4,       brute-force induced 4/5-vertex subgraph classification
+ *   orc_pattern       (networkx isomorphism) and the identities listed next
+ *                     to each pattern below                   tests/test_oracle.py
+ *   orc_ceci          the reference's 200 known answers (sub_ceci/test/
+ *                     expected_output.res on HPRD), networkx subgraph
+ *                     monomorphism counts, and the reference CECI binary
+ *                     built from source (oracle/_ref)         tests/test_ceci.py
+ *   orc_match_dfs     same checks as orc_ceci
+ */
+#include <omp.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <queue>
+#include <unordered_map>
+#include <vector>
+
+typedef uint32_t u32;
+typedef uint64_t u64;
+
+namespace {
+
+/* Sorted-set primitives (the semantics of src/graph/set_intersection.cuh and
+ * set_difference.cuh, done by a sequential merge / galloping search).       */
+
+inline bool contains(const u32* a, u32 na, u32 key) {
+  return std::binary_search(a, a + na, key);
+}
+
+// |A & B|, optional strict upper bound on the kept elements.
+inline u64 isect_count(const u32* a, u32 na, const u32* b, u32 nb,
+                       u32 ub = 0xFFFFFFFFu) {
+  if (na > nb) {
+    std::swap(a, b);
+    std::swap(na, nb);
+  }
+  u64 c = 0;
+  if (na == 0) return 0;
+  if ((u64)na * 32 < nb) {  // skewed sizes: binary search the small side
+    for (u32 i = 0; i < na && a[i] < ub; ++i) c += contains(b, nb, a[i]);
+    return c;
+  }
+  u32 i = 0, j = 0;
+  while (i < na && j < nb) {
+    u32 x = a[i], y = b[j];
+    if (x >= ub || y >= ub) break;
+    if (x < y)
+      ++i;
+    else if (y < x)
+      ++j;
+    else {
+      ++c;
+      ++i;
+      ++j;
+    }
+  }
+  return c;
+}
+
+// A & B materialised (sorted), optional strict upper bound.
+inline u32 isect(const u32* a, u32 na, const u32* b, u32 nb, u32* out,
+                 u32 ub = 0xFFFFFFFFu) {
+  u32 i = 0, j = 0, k = 0;
+  while (i < na && j < nb) {
+    u32 x = a[i], y = b[j];
+    if (x >= ub || y >= ub) break;
+    if (x < y)
+      ++i;
+    else if (y < x)
+      ++j;
+    else {
+      out[k++] = x;
+      ++i;
+      ++j;
+    }
+  }
+  return k;
+}
+
+// A \ B materialised (A iterated, no swap: set_difference.cuh:98-234),
+// optional strict upper bound on the kept elements.
+inline u32 diff(const u32* a, u32 na, const u32* b, u32 nb, u32* out,
+                u32 ub = 0xFFFFFFFFu) {
+  u32 i = 0, j = 0, k = 0;
+  while (i < na && a[i] < ub) {
+    u32 x = a[i];
+    while (j < nb && b[j] < x) ++j;
+    if (j >= nb || b[j] != x) out[k++] = x;
+    ++i;
+  }
+  return k;
+}
+
+// |A \ B| with optional upper bound (set_difference.cuh:6-96).
+inline u64 diff_count(const u32* a, u32 na, const u32* b, u32 nb,
+                      u32 ub = 0xFFFFFFFFu) {
+  u32 i = 0, j = 0;
+  u64 k = 0;
+  while (i < na && a[i] < ub) {
+    u32 x = a[i];
+    while (j < nb && b[j] < x) ++j;
+    if (j >= nb || b[j] != x) ++k;
+    ++i;
+  }
+  return k;
+}
+
+struct Csr {
+  u32 n;
+  const u32* off;
+  const u32* adj;
+  const u32* N(u32 v) const { return adj + off[v]; }
+  u32 d(u32 v) const { return off[v + 1] - off[v]; }
+};
+
+/* Nested clique plan (CFSolver.cuh:15-97 generalised to any depth):
+ * cnt(W, r) = number of r-cliques of the DAG inside W.                      */
+u64 clique_rec(const Csr& g, const u32* W, u32 nw, int r,
+               std::vector<std::vector<u32>>& buf, int depth, u64* bytes) {
+  if (r == 1) return nw;
+  u64 total = 0;
+  if (r == 2) {
+    for (u32 i = 0; i < nw; ++i) {
+      u32 v = W[i];
+      total += isect_count(W, nw, g.N(v), g.d(v));
+      if (bytes) *bytes += 4ull * ((u64)nw + g.d(v));
+    }
+    return total;
+  }
+  std::vector<u32>& out = buf[depth];
+  for (u32 i = 0; i < nw; ++i) {
+    u32 v = W[i];
+    u32 k = isect(W, nw, g.N(v), g.d(v), out.data());
+    if (bytes) *bytes += 4ull * ((u64)nw + g.d(v) + k);
+    // Copy: out is reused by the next sibling.
+    std::vector<u32> w2(out.begin(), out.begin() + k);
+    total += clique_rec(g, w2.data(), k, r - 1, buf, depth + 1, bytes);
+  }
+  return total;
+}
+
+u64 clique_roots(const Csr& g, int k, u32 rb, u32 re, u64* bytes_out) {
+  u32 maxd = 0;
+  for (u32 v = 0; v < g.n; ++v) maxd = std::max(maxd, g.d(v));
+  u64 total = 0, bytes = 0;
+#pragma omp parallel reduction(+ : total, bytes)
+  {
+    std::vector<std::vector<u32>> buf(k + 1, std::vector<u32>(maxd + 1));
+    std::vector<u32> w1(maxd + 1);
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t v0 = rb; v0 < (int64_t)re; ++v0) {
+      const u32* a = g.N((u32)v0);
+      u32 na = g.d((u32)v0);
+      for (u32 i = 0; i < na; ++i) {
+        u32 v1 = a[i];
+        if (k == 3) {
+          total += isect_count(a, na, g.N(v1), g.d(v1));
+          if (bytes_out) bytes += 4ull * ((u64)na + g.d(v1));
+          continue;
+        }
+        u32 nw = isect(a, na, g.N(v1), g.d(v1), w1.data());
+        if (bytes_out) bytes += 4ull * ((u64)na + g.d(v1) + nw);
+        total += clique_rec(g, w1.data(), nw, k - 2, buf, 0,
+                            bytes_out ? &bytes : nullptr);
+      }
+    }
+  }
+  if (bytes_out) *bytes_out = bytes;
+  return total;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_num_threads(void) { return omp_get_max_threads(); }
+void orc_set_threads(int t) { omp_set_num_threads(t > 0 ? t : 1); }
+
+/* Triangle count of a degree-oriented DAG: sum over oriented edges (u, v) of
+ * |N+(u) & N+(v)|, the e_centric count of src/TCSolver.cuh:27-59.            */
+u64 orc_tc(u32 n, const u32* off, const u32* adj) {
+  Csr g{n, off, adj};
+  return clique_roots(g, 3, 0, n, nullptr);
+}
+
+/* k-clique count (k >= 3) of a DAG restricted to root vertices v0 in
+ * [rb, re) -- src/CFSolver.cuh:15-97 (k = 4, 5) generalised.                 */
+u64 orc_clique(u32 n, const u32* off, const u32* adj, int k, u32 rb, u32 re) {
+  if (k < 3) return 0;
+  Csr g{n, off, adj};
+  if (re > n || re == 0) re = n;
+  return clique_roots(g, k, rb, re, nullptr);
+}
+
+/* Algorithmic bytes of the canonical plan: 4(n+1) + 8|E+| + sum over set ops
+ * of 4(|A|+|B|) (+ 4|out| for materialised results).                        */
+u64 orc_plan_bytes(u32 n, const u32* off, const u32* adj, int k) {
+  Csr g{n, off, adj};
+  u64 bytes = 0;
+  clique_roots(g, k, 0, n, &bytes);
+  return bytes + 4ull * (n + 1) + 8ull * off[n];
+}
+
+/* Motif4 counts on a symmetric graph, restating Motif4_G2Miner
+ * (src/gpu_kernels/G2Miner_kernels.cuh:438-552):
+ *   out[0] 3-star   : all directed (v0,v1): L = {x in N(v0)\N(v1), x < v1};
+ *                     sum_{v2 in L} |{x in L \ N(v2), x < v2}|          (:474-482)
+ *   out[4] diamond  : v1 < v0, I = N(v0)&N(v1): sum_{v2 in I}|{x in I\N(v2), x<v2}|
+ *   out[2] tailed   : sum_{v2 in I} |(N(v2) \ N(v0)) \ N(v1)|           (:488-504)
+ *   out[5] 4-clique : sum_{v2 in I, v2<v1} |{x in I & N(v2), x < v2}|    (:506-516)
+ *   out[1] 4-path   : sum_{v2 in N(v0)\N(v1)} |(N(v1)\N(v0)) \ N(v2)|    (:518-532)
+ *   out[3] 4-cycle  : A = {x in N(v1)\N(v0), x < v0}, B = {y in N(v0)\N(v1),
+ *                     y < v1}: sum_{v2 in B} |A & N(v2)|                 (:534-548)
+ * Each equals the number of induced copies of that motif (pinned by brute
+ * force in the tests).                                                       */
+void orc_motif4(u32 n, const u32* off, const u32* adj, u64* out) {
+  Csr g{n, off, adj};
+  u32 maxd = 0;
+  for (u32 v = 0; v < n; ++v) maxd = std::max(maxd, g.d(v));
+  u64 c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
+#pragma omp parallel reduction(+ : c0, c1, c2, c3, c4, c5)
+  {
+    std::vector<u32> L(maxd + 1), I(maxd + 1), A(maxd + 1), B(maxd + 1),
+        T(maxd + 1);
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t v0i = 0; v0i < (int64_t)n; ++v0i) {
+      u32 v0 = (u32)v0i;
+      const u32* N0 = g.N(v0);
+      u32 d0 = g.d(v0);
+      for (u32 e = 0; e < d0; ++e) {
+        u32 v1 = N0[e];
+        const u32* N1 = g.N(v1);
+        u32 d1 = g.d(v1);
+        u32 nl = diff(N0, d0, N1, d1, L.data(), v1);
+        for (u32 i = 0; i < nl; ++i) {
+          u32 v2 = L[i];
+          c0 += diff_count(L.data(), nl, g.N(v2), g.d(v2), v2);
+        }
+        if (v1 > v0) continue;
+        u32 ni = isect(N0, d0, N1, d1, I.data());
+        for (u32 i = 0; i < ni; ++i) {
+          u32 v2 = I[i];
+          c4 += diff_count(I.data(), ni, g.N(v2), g.d(v2), v2);
+          u32 nt = diff(g.N(v2), g.d(v2), N0, d0, T.data());
+          c2 += diff_count(T.data(), nt, N1, d1);
+        }
+        u32 nb = isect(N0, d0, N1, d1, B.data(), v1);
+        for (u32 i = 0; i < nb; ++i) {
+          u32 v2 = B[i];
+          c5 += isect_count(B.data(), nb, g.N(v2), g.d(v2), v2);
+        }
+        u32 na = diff(N0, d0, N1, d1, A.data());
+        nb = diff(N1, d1, N0, d0, B.data());
+        for (u32 i = 0; i < na; ++i) {
+          u32 v2 = A[i];
+          c1 += diff_count(B.data(), nb, g.N(v2), g.d(v2));
+        }
+        na = diff(N1, d1, N0, d0, A.data(), v0);
+        nb = diff(N0, d0, N1, d1, B.data(), v1);
+        for (u32 i = 0; i < nb; ++i) {
+          u32 v2 = B[i];
+          c3 += isect_count(A.data(), na, g.N(v2), g.d(v2), v0);
+        }
+      }
+    }
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+  out[4] = c4;
+  out[5] = c5;
+}
+
+/* Fixed patterns P1..P8 on a symmetric graph, restating the G2Miner kernels
+ * (src/gpu_kernels/G2Miner_kernels.cuh).  In terms of induced subgraphs:
+ *   P1 = #3-star (claw)                                          (:6-50)
+ *   P2 = #chair (5 vertices, degrees 3,2,1,1,1)                  (:51-108)
+ *   P3 = 3 #claw + 4 #K1,4                                       (:109-162)
+ *   P4 = #4-path (v1 < v0 only)                                  (:163-199)
+ *   P5 = 2 #bull                                                 (:201-262)
+ *   P6 = 2 #tailed-triangle + 2 #(diamond + pendant at a degree-3 vertex)
+ *                                                                (:263-319)
+ *   P7 = 2 #K1,1,3                                               (:320-375)
+ *   P8 = 3 #(K4 + pendant)                                       (:377-435)
+ * Returns the value of counters[0] (G2MinerSolver.cuh:166).                  */
+u64 orc_pattern(u32 n, const u32* off, const u32* adj, int pid) {
+  Csr g{n, off, adj};
+  u32 maxd = 0;
+  for (u32 v = 0; v < n; ++v) maxd = std::max(maxd, g.d(v));
+  u64 total = 0;
+#pragma omp parallel reduction(+ : total)
+  {
+    std::vector<u32> L0(maxd + 1), L1(maxd + 1), L2(maxd + 1), L3(maxd + 1),
+        L4(maxd + 1);
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t v0i = 0; v0i < (int64_t)n; ++v0i) {
+      u32 v0 = (u32)v0i;
+      const u32* N0 = g.N(v0);
+      u32 d0 = g.d(v0);
+      for (u32 e = 0; e < d0; ++e) {
+        u32 v1 = N0[e];
+        const u32* N1 = g.N(v1);
+        u32 d1 = g.d(v1);
+        switch (pid) {
+          case 1: {  // G2Miner_kernels.cuh:22-42
+            u32 n0 = diff(N0, d0, N1, d1, L0.data(), v1);
+            for (u32 i = 0; i < n0; ++i) {
+              u32 v2 = L0[i];
+              total += diff_count(L0.data(), n0, g.N(v2), g.d(v2), v2);
+            }
+          } break;
+          case 2: {  // :68-103
+            u32 n0 = diff(N0, d0, N1, d1, L0.data());
+            u32 n1 = diff(N1, d1, N0, d0, L1.data());
+            for (u32 i = 0; i < n1; ++i) {
+              u32 v4 = L1[i];
+              u32 n2 = diff(L0.data(), n0, g.N(v4), g.d(v4), L2.data());
+              for (u32 j = 0; j < n2; ++j) {
+                u32 v2 = L2[j];
+                total += diff_count(L2.data(), n2, g.N(v2), g.d(v2), v2);
+              }
+            }
+          } break;
+          case 3: {  // :126-157
+            u32 n0 = diff(N0, d0, N1, d1, L0.data());
+            for (u32 i = 0; i < n0; ++i) {
+              u32 v2 = L0[i];
+              u32 n1 = diff(L0.data(), n0, g.N(v2), g.d(v2), L1.data(), v2);
+              for (u32 j = 0; j < n1; ++j) {
+                u32 v3 = L1[j];
+                total += diff_count(L1.data(), n1, g.N(v3), g.d(v3), v3);
+              }
+            }
+          } break;
+          case 4: {  // :174-197
+            if (v1 >= v0) break;
+            u32 n0 = diff(N0, d0, N1, d1, L0.data());
+            u32 n1 = diff(N1, d1, N0, d0, L1.data());
+            for (u32 j = 0; j < n0; ++j) {
+              u32 v2 = L0[j];
+              total += diff_count(L1.data(), n1, g.N(v2), g.d(v2));
+            }
+          } break;
+          case 5: {  // :219-259
+            u32 n0 = diff(N0, d0, N1, d1, L0.data());
+            u32 n1 = diff(N1, d1, N0, d0, L1.data());
+            u32 n2 = isect(N0, d0, N1, d1, L2.data());
+            for (u32 i = 0; i < n2; ++i) {
+              u32 v2 = L2[i];
+              u32 n3 = diff(L0.data(), n0, g.N(v2), g.d(v2), L3.data());
+              u32 n4 = diff(L1.data(), n1, g.N(v2), g.d(v2), L4.data());
+              for (u32 j = 0; j < n3; ++j) {
+                u32 v3 = L3[j];
+                total += diff_count(L4.data(), n4, g.N(v3), g.d(v3));
+              }
+            }
+          } break;
+          case 6: {  // :281-316
+            u32 n0 = isect(N0, d0, N1, d1, L0.data());
+            u32 n1 = diff(N1, d1, N0, d0, L1.data());
+            for (u32 i = 0; i < n0; ++i) {
+              u32 v2 = L0[i];
+              u32 n2 = diff(L0.data(), n0, g.N(v2), g.d(v2), L2.data());
+              u32 n3 = diff(L1.data(), n1, g.N(v2), g.d(v2), L3.data());
+              for (u32 j = 0; j < n2; ++j) {
+                u32 v3 = L2[j];
+                total += diff_count(L3.data(), n3, g.N(v3), g.d(v3));
+              }
+            }
+          } break;
+          case 7: {  // :339-370
+            u32 n0 = isect(N0, d0, N1, d1, L0.data());
+            for (u32 i = 0; i < n0; ++i) {
+              u32 v2 = L0[i];
+              u32 n1 = diff(L0.data(), n0, g.N(v2), g.d(v2), L1.data(), v2);
+              for (u32 j = 0; j < n1; ++j) {
+                u32 v3 = L1[j];
+                total += diff_count(L1.data(), n1, g.N(v3), g.d(v3), v3);
+              }
+            }
+          } break;
+          case 8: {  // :395-429
+            u32 n0 = isect(N0, d0, N1, d1, L0.data());
+            u32 n1 = diff(N1, d1, N0, d0, L1.data());
+            for (u32 i = 0; i < n1; ++i) {
+              u32 v4 = L1[i];
+              u32 n2 = diff(L0.data(), n0, g.N(v4), g.d(v4), L2.data());
+              for (u32 j = 0; j < n2; ++j) {
+                u32 v3 = L2[j];
+                total += isect_count(L2.data(), n2, g.N(v3), g.d(v3), v3);
+              }
+            }
+          } break;
+          default:
+            break;
+        }
+      }
+    }
+  }
+  return total;
+}
+
+/* ------------------------------------------------------------------ CECI */
+
+/* CECI restatement (sub_ceci/myceci.cpp).  Data graph: symmetric CSR with
+ * sorted lists, labels[n].  Query: qn vertices, qm undirected edges given as
+ * 2*qm endpoints, qlabels[qn].  Returns the number of injective, label- and
+ * edge-preserving maps (non-induced; automorphisms counted), the value the
+ * reference prints as "#Embeddings" (myceci.cpp:1115).
+ *
+ * Steps restated:
+ *   candidates with label + degree + NLF filter    myceci.cpp:120-169
+ *   start vertex argmin |cand|/deg                 myceci.cpp:15-31
+ *   BFS tree, backward (non-tree) neighbours       myceci.cpp:33-103
+ *   TE / NTE candidate maps (parent cand -> sorted
+ *   child candidates), empty-map pruning           myceci.cpp:213-369
+ *   DFS: TE(parent) intersected with NTE(each
+ *   backward nbr), visited check                   myceci.cpp:540-724
+ * The reverse-BFS cardinality refinement (myceci.cpp:370-466) only removes
+ * candidates that extend to no embedding, so it is left out: counts are
+ * unchanged by it.                                                           */
+namespace {
+
+struct QueryG {
+  u32 n;
+  std::vector<std::vector<u32>> adj;
+  std::vector<u32> lab;
+};
+
+bool nlf_ok(const Csr& g, const u32* labels, u32 v, const QueryG& q, u32 u,
+            std::vector<u32>& scratch_cnt, u32 n_labels) {
+  // Count per label of u's query neighbours vs. v's data neighbours.
+  std::unordered_map<u32, u32> need;
+  for (u32 w : q.adj[u]) need[q.lab[w]]++;
+  std::unordered_map<u32, u32> have;
+  const u32* N = g.N(v);
+  for (u32 i = 0; i < g.d(v); ++i) {
+    u32 l = labels[N[i]];
+    if (need.count(l)) have[l]++;
+  }
+  for (auto& kv : need) {
+    auto it = have.find(kv.first);
+    if (it == have.end() || it->second < kv.second) return false;
+  }
+  (void)scratch_cnt;
+  (void)n_labels;
+  return true;
+}
+
+u64 ceci_count(const Csr& g, const u32* labels, const QueryG& q) {
+  const u32 qn = q.n;
+  if (qn == 0) return 0;
+  std::vector<u32> scratch;
+  auto passes = [&](u32 v, u32 u) {
+    return labels[v] == q.lab[u] && g.d(v) >= q.adj[u].size() &&
+           nlf_ok(g, labels, v, q, u, scratch, 0);
+  };
+  // Start vertex: argmin |candidates(u)| / deg(u)  (myceci.cpp:15-31).
+  std::vector<std::vector<u32>> cand(qn);
+  u32 start = 0;
+  double best = (double)g.n;
+  for (u32 u = 0; u < qn; ++u) {
+    for (u32 v = 0; v < g.n; ++v)
+      if (passes(v, u)) cand[u].push_back(v);
+    double deg = (double)q.adj[u].size();
+    double score = deg > 0 ? cand[u].size() / deg : (double)cand[u].size();
+    if (score < best) {
+      best = score;
+      start = u;
+    }
+  }
+  // BFS tree (myceci.cpp:65-103).
+  std::vector<u32> order, parent(qn, 0xFFFFFFFFu), pos(qn, 0);
+  std::vector<char> seen(qn, 0);
+  std::queue<u32> bq;
+  bq.push(start);
+  seen[start] = 1;
+  while (!bq.empty()) {
+    u32 u = bq.front();
+    bq.pop();
+    pos[u] = (u32)order.size();
+    order.push_back(u);
+    for (u32 w : q.adj[u])
+      if (!seen[w]) {
+        seen[w] = 1;
+        parent[w] = u;
+        bq.push(w);
+      }
+  }
+  if (order.size() != qn) return 0;  // disconnected query
+  // Backward non-tree neighbours (myceci.cpp:45-57).
+  std::vector<std::vector<u32>> bn(qn);
+  for (u32 i = 0; i < qn; ++i) {
+    u32 u = order[i];
+    for (u32 w : q.adj[u])
+      if (w != parent[u] && pos[w] < pos[u]) bn[u].push_back(w);
+  }
+  // TE / NTE maps: for query vertex u and neighbour-position p, map from the
+  // data vertex matched to p to the sorted list of u's candidates adjacent to
+  // it.  The root keeps its filtered candidate list.
+  std::vector<char> is_cand(g.n, 0);
+  std::vector<std::unordered_map<u32, std::vector<u32>>> te(qn);
+  std::vector<std::vector<std::unordered_map<u32, std::vector<u32>>>> nte(qn);
+  for (u32 i = 1; i < qn; ++i) {
+    u32 u = order[i];
+    for (u32 v : cand[u]) is_cand[v] = 1;
+    auto build = [&](u32 up, std::unordered_map<u32, std::vector<u32>>& map) {
+      for (u32 vf : cand[up]) {
+        std::vector<u32> lst;
+        const u32* N = g.N(vf);
+        for (u32 k = 0; k < g.d(vf); ++k)
+          if (is_cand[N[k]]) lst.push_back(N[k]);
+        if (!lst.empty()) map.emplace(vf, std::move(lst));
+      }
+    };
+    build(parent[u], te[u]);
+    nte[u].resize(bn[u].size());
+    for (size_t j = 0; j < bn[u].size(); ++j) build(bn[u][j], nte[u][j]);
+    for (u32 v : cand[u]) is_cand[v] = 0;
+  }
+  // Iterative DFS (myceci.cpp:540-668) with the candidate intersection of
+  // generateValidCandidates (myceci.cpp:670-724).
+  std::vector<u32> emb(qn, 0);
+  std::vector<char> used(g.n, 0);
+  std::vector<std::vector<u32>> valid(qn);
+  std::vector<u32> idx(qn, 0);
+  std::vector<u32> tmp;
+  u64 count = 0;
+  valid[0] = cand[start];
+  int depth = 0;
+  idx[0] = 0;
+  auto gen = [&](int d) {
+    u32 u = order[d];
+    valid[d].clear();
+    auto it = te[u].find(emb[parent[u]]);
+    if (it == te[u].end()) return;
+    valid[d] = it->second;
+    for (size_t j = 0; j < bn[u].size() && !valid[d].empty(); ++j) {
+      auto jt = nte[u][j].find(emb[bn[u][j]]);
+      if (jt == nte[u][j].end()) {
+        valid[d].clear();
+        return;
+      }
+      tmp.resize(std::min(valid[d].size(), jt->second.size()));
+      u32 k = isect(valid[d].data(), (u32)valid[d].size(), jt->second.data(),
+                    (u32)jt->second.size(), tmp.data());
+      valid[d].assign(tmp.begin(), tmp.begin() + k);
+    }
+  };
+  while (true) {
+    while (idx[depth] < valid[depth].size()) {
+      u32 u = order[depth];
+      u32 v = valid[depth][idx[depth]++];
+      if (used[v]) continue;
+      if (depth == (int)qn - 1) {
+        ++count;
+        continue;
+      }
+      emb[u] = v;
+      used[v] = 1;
+      ++depth;
+      idx[depth] = 0;
+      gen(depth);
+    }
+    --depth;
+    if (depth < 0) break;
+    used[emb[order[depth]]] = 0;
+  }
+  return count;
+}
+
+QueryG make_query(u32 qn, u32 qm, const u32* edges, const u32* qlabels) {
+  QueryG q;
+  q.n = qn;
+  q.adj.assign(qn, {});
+  q.lab.assign(qlabels, qlabels + qn);
+  for (u32 i = 0; i < qm; ++i) {
+    u32 a = edges[2 * i], b = edges[2 * i + 1];
+    if (a == b || a >= qn || b >= qn) continue;
+    q.adj[a].push_back(b);
+    q.adj[b].push_back(a);
+  }
+  for (auto& l : q.adj) {
+    std::sort(l.begin(), l.end());
+    l.erase(std::unique(l.begin(), l.end()), l.end());
+  }
+  return q;
+}
+
+}  // namespace
+
+u64 orc_ceci(u32 n, const u32* off, const u32* adj, const u32* labels, u32 qn,
+             u32 qm, const u32* qedges, const u32* qlabels) {
+  Csr g{n, off, adj};
+  QueryG q = make_query(qn, qm, qedges, qlabels);
+  return ceci_count(g, labels, q);
+}
+
+/* Many queries, parallel across queries (one thread per query).             */
+void orc_ceci_batch(u32 n, const u32* off, const u32* adj, const u32* labels,
+                    int nq, const u32* qn, const u32* qm,
+                    const u64* qedge_off, const u32* qedges,
+                    const u64* qlab_off, const u32* qlabels, u64* out) {
+  Csr g{n, off, adj};
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int i = 0; i < nq; ++i) {
+    QueryG q = make_query(qn[i], qm[i], qedges + qedge_off[i],
+                          qlabels + qlab_off[i]);
+    out[i] = ceci_count(g, labels, q);
+  }
+}
+
+}  // extern "C"

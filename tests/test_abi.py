@@ -1,0 +1,34 @@
+"""The C-ABI library loads and exports every symbol include/gpm.h declares
+(no compute calls: no GPU needed)."""
+import ctypes
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "gpm.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(gpm_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_core_api():
+    names = declared_functions()
+    for must in ("gpm_graph_create", "gpm_tc", "gpm_clique", "gpm_pattern",
+                 "gpm_match_batch", "gpm_last_error", "gpm_mlp_predict"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol(gpm):
+    lib = gpm.lib()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert missing == []
+
+
+def test_error_path_without_gpu(gpm):
+    # Argument errors are reported through status + message, not aborts.
+    lib = gpm.lib()
+    st = lib.gpm_csr_orient(ctypes.c_uint32(0), None, None, None, None, None)
+    assert st == 1
+    assert b"null" in lib.gpm_last_error()
