@@ -46,6 +46,22 @@ __device__ __forceinline__ bool use_probe(uint32_t len, uint32_t d) {
 
 __device__ __forceinline__ uint32_t ctz64(uint64_t x) { return (uint32_t)__builtin_ctzll(x); }
 
+// Position of the k-th (0-based) set bit of x; x must have more than k bits.
+__device__ __forceinline__ uint32_t select_bit(uint64_t x, uint32_t k) {
+  uint32_t pos = 0;
+#pragma unroll
+  for (uint32_t half = 32; half >= 1; half >>= 1) {
+    const uint64_t lo_mask = (half == 64) ? ~0ull : ((1ull << half) - 1);
+    const uint32_t c = (uint32_t)__popcll(x & lo_mask);
+    if (k >= c) {
+      k -= c;
+      x >>= half;
+      pos += half;
+    }
+  }
+  return pos;
+}
+
 // ------------------------------------------------------------ wave tier
 template <int K, int MODE>
 __global__ __launch_bounds__(kBlock) void clique_wave_kernel(
@@ -54,7 +70,7 @@ __global__ __launch_bounds__(kBlock) void clique_wave_kernel(
     unsigned int* __restrict__ queue, unsigned long long* __restrict__ out,
     unsigned long long* __restrict__ work) {
   __shared__ uint32_t s_keys[kWavesPerBlock][kWaveTable];
-  __shared__ uint16_t s_vals[kWavesPerBlock][kWaveTable];
+  __shared__ uint32_t s_vals[kWavesPerBlock][kWaveTable];
   __shared__ unsigned long long s_rows[kWavesPerBlock][kWave];
   __shared__ uint32_t s_pre[kWavesPerBlock][kWave];
   __shared__ uint32_t s_beg[kWavesPerBlock][kWave];
@@ -62,7 +78,7 @@ __global__ __launch_bounds__(kBlock) void clique_wave_kernel(
   const uint32_t lane = lane_id();
   const uint32_t w = threadIdx.x / kWave;
   uint32_t* keys = s_keys[w];
-  uint16_t* vals = s_vals[w];
+  uint32_t* vals = s_vals[w];
   unsigned long long* rows = s_rows[w];
   uint32_t* pre = s_pre[w];
   uint32_t* beg = s_beg[w];
@@ -76,15 +92,15 @@ __global__ __launch_bounds__(kBlock) void clique_wave_kernel(
     if (base >= n_roots) break;
     const uint32_t stop = min(base + chunk, n_roots);
     for (uint32_t r = base; r < stop; ++r) {
-      const uint32_t u = roots[r];
-      const uint32_t sb = off[u], d = off[u + 1] - sb;  // d <= 64 by tiering
+      const uint32_t u = uniform(roots[r]);
+      const uint32_t sb = uniform(off[u]), d = uniform(off[u + 1]) - sb;  // d <= 64 by tiering
       const bool act = lane < d;
       const uint32_t s = act ? adj[sb + lane] : kEmpty;
       keys[lane] = kEmpty;
       keys[lane + kWave] = kEmpty;
       rows[lane] = 0ull;
       __builtin_amdgcn_wave_barrier();
-      if (act) lds_hash_insert(keys, vals, hmask, s, (uint16_t)lane);
+      if (act) lds_hash_insert(keys, vals, hmask, s, lane);
       uint32_t b = 0, len = 0;
       if (act) {
         b = off[s];
@@ -123,19 +139,30 @@ __global__ __launch_bounds__(kBlock) void clique_wave_kernel(
         if (K == 3) {
           c = __popcll(row);
         } else {
-          uint64_t bits = row;
-          while (bits) {
-            const uint32_t j = ctz64(bits);
-            bits &= bits - 1;
-            const uint64_t rj = row & rows[j];
-            if (K == 4) {
-              c += __popcll(rj);
-            } else {
-              uint64_t mm = rj;
-              while (mm) {
-                const uint32_t l = ctz64(mm);
-                mm &= mm - 1;
-                c += __popcll(rj & rows[l]);
+          // The (i, j in row_i) pairs are dealt out evenly over the lanes:
+          // pair p belongs to the row whose prefix range holds p, and j is
+          // the (p - start)-th set bit of that row.
+          const uint32_t pincl = wave_incl_scan((uint32_t)__popcll(row));
+          pre[lane] = pincl;
+          const uint32_t npairs = wave_bcast(pincl, kWave - 1);
+          __builtin_amdgcn_wave_barrier();
+          for (uint32_t t = 0; t < npairs; t += kWave) {
+            const uint32_t p = t + lane;
+            if (p < npairs) {
+              const uint32_t i = upper_index(pre, kWave, p);
+              const uint32_t excl = i ? pre[i - 1] : 0u;
+              const uint64_t ri = rows[i];
+              const uint32_t j = select_bit(ri, p - excl);
+              const uint64_t mij = ri & rows[j];
+              if (K == 4) {
+                c += __popcll(mij);
+              } else {
+                uint64_t mm = mij;
+                while (mm) {
+                  const uint32_t l = ctz64(mm);
+                  mm &= mm - 1;
+                  c += __popcll(mij & rows[l]);
+                }
               }
             }
           }
@@ -195,6 +222,72 @@ __device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t* s_wsum
 }
 
 // Per-root counting (or accounting) over W-word rows; thread-owned rows.
+// K=4/5 counting with the (i, j in row_i) pairs dealt out evenly over the
+// block: rows are taken kBlock at a time, a block scan of their popcounts
+// gives each pair an owner row, and j is found by popcount over the words.
+// MAXW > 0 keeps row_i & row_j in registers (W <= MAXW).
+template <int K, int MAXW>
+__device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, uint32_t d,
+                                                uint32_t W, uint32_t* s_pre,
+                                                uint32_t* s_wsum) {
+  uint64_t c = 0;
+  for (uint32_t c0 = 0; c0 < d; c0 += kBlock) {
+    const uint32_t i0 = c0 + threadIdx.x;
+    uint32_t pc = 0;
+    if (i0 < d)
+      for (uint32_t w = 0; w < W; ++w) pc += (uint32_t)__popcll(rows[(size_t)i0 * W + w]);
+    const uint32_t incl = block_incl_scan(pc, s_wsum);
+    s_pre[threadIdx.x] = incl;
+    __syncthreads();
+    const uint32_t total = uniform(s_pre[kBlock - 1]);
+    for (uint32_t p = threadIdx.x; p < total; p += kBlock) {
+      const uint32_t jj = upper_index(s_pre, kBlock, p);
+      uint32_t kth = p - (jj ? s_pre[jj - 1] : 0u);
+      const unsigned long long* ri = rows + (size_t)(c0 + jj) * W;
+      uint32_t w = 0;
+      while (true) {
+        const uint32_t pw = (uint32_t)__popcll(ri[w]);
+        if (kth < pw) break;
+        kth -= pw;
+        ++w;
+      }
+      const uint32_t j = w * 64 + select_bit(ri[w], kth);
+      const unsigned long long* rj = rows + (size_t)j * W;
+      if (K == 4) {
+        for (uint32_t x = 0; x < W; ++x) c += __popcll(ri[x] & rj[x]);
+      } else if (MAXW > 0) {
+        uint64_t m[MAXW > 0 ? MAXW : 1];
+#pragma unroll
+        for (int x = 0; x < MAXW; ++x) m[x] = (x < (int)W) ? (ri[x] & rj[x]) : 0ull;
+#pragma unroll
+        for (int x = 0; x < MAXW; ++x) {
+          uint64_t mm = m[x];
+          while (mm) {
+            const uint32_t l = x * 64 + ctz64(mm);
+            mm &= mm - 1;
+            const unsigned long long* rl = rows + (size_t)l * W;
+#pragma unroll
+            for (int y = 0; y < MAXW; ++y)
+              if (y < (int)W) c += __popcll(m[y] & rl[y]);
+          }
+        }
+      } else {
+        for (uint32_t x = 0; x < W; ++x) {
+          uint64_t mm = ri[x] & rj[x];
+          while (mm) {
+            const uint32_t l = x * 64 + ctz64(mm);
+            mm &= mm - 1;
+            const unsigned long long* rl = rows + (size_t)l * W;
+            for (uint32_t y = 0; y < W; ++y) c += __popcll(ri[y] & rj[y] & rl[y]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  return c;
+}
+
 template <int K, int MODE>
 __device__ __forceinline__ uint64_t count_rows(const unsigned long long* rows, uint32_t d,
                                                uint32_t W, const uint32_t* lens) {
@@ -202,31 +295,7 @@ __device__ __forceinline__ uint64_t count_rows(const unsigned long long* rows, u
   for (uint32_t i = threadIdx.x; i < d; i += kBlock) {
     const unsigned long long* ri = rows + (size_t)i * W;
     if (MODE == kCount) {
-      if (K == 3) {
-        for (uint32_t w = 0; w < W; ++w) c += __popcll(ri[w]);
-        continue;
-      }
-      for (uint32_t w = 0; w < W; ++w) {
-        uint64_t bits = ri[w];
-        while (bits) {
-          const uint32_t j = w * 64 + ctz64(bits);
-          bits &= bits - 1;
-          const unsigned long long* rj = rows + (size_t)j * W;
-          if (K == 4) {
-            for (uint32_t x = 0; x < W; ++x) c += __popcll(ri[x] & rj[x]);
-          } else {
-            for (uint32_t x = 0; x < W; ++x) {
-              uint64_t mm = ri[x] & rj[x];
-              while (mm) {
-                const uint32_t l = x * 64 + ctz64(mm);
-                mm &= mm - 1;
-                const unsigned long long* rl = rows + (size_t)l * W;
-                for (uint32_t y = 0; y < W; ++y) c += __popcll(ri[y] & rj[y] & rl[y]);
-              }
-            }
-          }
-        }
-      }
+      for (uint32_t w = 0; w < W; ++w) c += __popcll(ri[w]);  // K == 3 only
     } else {
       uint64_t w1 = 0;
       for (uint32_t w = 0; w < W; ++w) w1 += __popcll(ri[w]);
@@ -263,9 +332,9 @@ __device__ __forceinline__ uint64_t count_rows(const unsigned long long* rows, u
 }
 
 // Block per root.  LDS_ROWS: rows, S, lens, begins and the hash table live in
-// dynamic LDS (|S| <= cap <= kBlockCap).  Otherwise rows live in this block's
+// dynamic LDS (|S| <= cap, W <= MAXW words per row).  Otherwise rows live in this block's
 // HBM region `grows` (cap*W words) and membership in S is a binary search.
-template <int K, int MODE, bool LDS_ROWS>
+template <int K, int MODE, bool LDS_ROWS, int MAXW>
 __global__ __launch_bounds__(kBlock) void clique_block_kernel(
     const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
     const uint32_t* __restrict__ roots, uint32_t n_roots, uint32_t cap,
@@ -281,13 +350,13 @@ __global__ __launch_bounds__(kBlock) void clique_block_kernel(
   const uint32_t Wcap = (cap + 63) / 64;
   unsigned long long* rows;
   uint32_t *lens, *begs, *keys;
-  uint16_t* vals;
+  uint32_t* vals;
   if (LDS_ROWS) {
     rows = s_dyn;
     lens = reinterpret_cast<uint32_t*>(rows + (size_t)cap * Wcap);
     begs = lens + cap;
     keys = begs + cap;
-    vals = reinterpret_cast<uint16_t*>(keys + table);
+    vals = keys + table;
   } else {
     rows = grows + (size_t)blockIdx.x * cap * Wcap;
     lens = gmeta + (size_t)blockIdx.x * 2 * cap;
@@ -300,11 +369,11 @@ __global__ __launch_bounds__(kBlock) void clique_block_kernel(
   while (true) {
     if (threadIdx.x == 0) s_root = atomicAdd(queue, 1u);
     __syncthreads();
-    const uint32_t r = s_root;
+    const uint32_t r = uniform(s_root);
     __syncthreads();
     if (r >= n_roots) break;
-    const uint32_t u = roots[r];
-    const uint32_t sb = off[u], d = off[u + 1] - sb;
+    const uint32_t u = uniform(roots[r]);
+    const uint32_t sb = uniform(off[u]), d = uniform(off[u + 1]) - sb;
     const uint32_t* S = adj + sb;
     const uint32_t W = (d + 63) / 64;
     if (threadIdx.x == 0) s_any_probe = 0;
@@ -323,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void clique_block_kernel(
     if (!LDS_ROWS) __threadfence();
     if (LDS_ROWS)
       for (uint32_t i = threadIdx.x; i < d; i += kBlock)
-        lds_hash_insert(keys, vals, hmask, S[i], (uint16_t)i);
+        lds_hash_insert(keys, vals, hmask, S[i], i);
     __syncthreads();
     // Flattened walk over the lists that are streamed.
     for (uint32_t c0 = 0; c0 < d; c0 += kBlock) {
@@ -339,7 +408,7 @@ __global__ __launch_bounds__(kBlock) void clique_block_kernel(
       const uint32_t incl = block_incl_scan(wl, s_wsum);
       s_pre[threadIdx.x] = incl;
       __syncthreads();
-      const uint32_t total = s_pre[kBlock - 1];
+      const uint32_t total = uniform(s_pre[kBlock - 1]);
       for (uint32_t f = threadIdx.x; f < total; f += kBlock) {
         const uint32_t j = upper_index(s_pre, kBlock, f);
         const uint32_t excl = j ? s_pre[j - 1] : 0u;
@@ -358,7 +427,7 @@ __global__ __launch_bounds__(kBlock) void clique_block_kernel(
     }
     // Probed lists: the ballot of "S[t] in N+(S[i])" over 64 consecutive t
     // is row word t/64.
-    if (s_any_probe) {
+    if (uniform((uint32_t)s_any_probe)) {
       for (uint32_t i = 0; i < d; ++i) {
         const uint32_t len = lens[i];
         if (!use_probe(len, d)) continue;
@@ -374,7 +443,9 @@ __global__ __launch_bounds__(kBlock) void clique_block_kernel(
     }
     __syncthreads();
     if (!LDS_ROWS) __threadfence();
-    uint64_t c = count_rows<K, MODE>(rows, d, W, lens);
+    uint64_t c = (MODE == kCount && K >= 4)
+                     ? count_pairs<K, MAXW>(rows, d, W, s_pre, s_wsum)
+                     : count_rows<K, MODE>(rows, d, W, lens);
     if (MODE == kWork) {
       c = wave_sum_u64(c);
       if (lane_id() == 0) s_red[threadIdx.x / kWave] = c;
@@ -402,7 +473,54 @@ __global__ void zero_u32_kernel(unsigned int* p, int count) {
 
 size_t block_lds_bytes(uint32_t cap, uint32_t table) {
   const uint32_t W = (cap + 63) / 64;
-  return (size_t)cap * W * 8 + (size_t)cap * 8 + (size_t)table * 4 + (size_t)table * 2;
+  return (size_t)cap * W * 8 + (size_t)cap * 8 + (size_t)table * 8;
+}
+
+// Largest out-degree whose rows fit one block's LDS on this device (at most
+// 1024 = 16 words per row); kBlockCap when the device allows only 64 KiB.
+uint32_t lds_wide_cap() {
+  static int cached = -1;
+  if (cached < 0) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    size_t limit = 64 * 1024;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      limit = prop.sharedMemPerBlock;
+    uint32_t cap = kBlockCap;
+    for (uint32_t c = 1024; c > kBlockCap; c -= 64) {
+      uint32_t table = 1;
+      while (table < 2 * c) table <<= 1;
+      if (block_lds_bytes(c, table) + 4096 <= limit) {
+        cap = c;
+        break;
+      }
+    }
+    cached = (int)cap;
+  }
+  return (uint32_t)cached;
+}
+
+// One LDS-rows tier: rows of up to MAXW words, cap = its largest out-degree.
+template <int K, int MODE, int MAXW>
+int launch_lds_tier(gpm_graph* g, const uint32_t* roots, uint32_t n_roots, uint32_t cap,
+                    unsigned int* queue, unsigned long long* out, unsigned long long* work,
+                    const char* name) {
+  const int cus = device_cus(g);
+  uint32_t table = 1;
+  while (table < 2 * cap) table <<= 1;
+  const size_t lds = block_lds_bytes(cap, table);
+  auto kern = clique_block_kernel<K, MODE, true, MAXW>;
+  if (lds > 48 * 1024)
+    GPM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const int per_cu = std::max<int>(1, (int)(160 * 1024 / (lds + 2048)));
+  const unsigned blocks =
+      (unsigned)std::min<uint64_t>(n_roots, (uint64_t)cus * std::min(per_cu, 8));
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), lds, g->stream, g->off, g->adj, roots,
+                     n_roots, cap, table, queue, nullptr, nullptr, out, work);
+  GPM_HIP(hipGetLastError());
+  if (MODE == kCount) timing_mark(g, name);
+  return GPM_OK;
 }
 
 template <int K, int MODE>
@@ -422,23 +540,21 @@ int launch_tiers(gpm_graph* g, const RootTiers* t, unsigned long long* out,
     if (st != GPM_OK) return st;
     unsigned long long* grows = static_cast<unsigned long long*>(g->scratch);
     uint32_t* gmeta = reinterpret_cast<uint32_t*>(static_cast<char*>(g->scratch) + row_bytes);
-    hipLaunchKernelGGL((clique_block_kernel<K, MODE, false>), dim3(blocks), dim3(kBlock), 0,
+    hipLaunchKernelGGL((clique_block_kernel<K, MODE, false, 0>), dim3(blocks), dim3(kBlock), 0,
                        g->stream, g->off, g->adj, t->d_big, t->n_big, cap, 0u, q + 2, grows,
                        gmeta, out, work);
     GPM_HIP(hipGetLastError());
+    if (MODE == kCount) timing_mark(g, "clique_hbm_block");
   }
   if (t->n_block) {
-    const uint32_t cap = t->max_block_deg;
-    uint32_t table = 1;
-    while (table < 2 * cap) table <<= 1;
-    const size_t lds = block_lds_bytes(cap, table);
-    const int per_cu = std::max<int>(1, (int)(160 * 1024 / (lds + 4096)));
-    const unsigned blocks =
-        (unsigned)std::min<uint64_t>(t->n_block, (uint64_t)cus * std::min(per_cu, 8));
-    hipLaunchKernelGGL((clique_block_kernel<K, MODE, true>), dim3(blocks), dim3(kBlock), lds,
-                       g->stream, g->off, g->adj, t->d_block, t->n_block, cap, table, q + 1,
-                       nullptr, nullptr, out, work);
-    GPM_HIP(hipGetLastError());
+    int st = launch_lds_tier<K, MODE, 8>(g, t->d_block, t->n_block, t->max_block_deg, q + 1,
+                                         out, work, "clique_lds_block");
+    if (st != GPM_OK) return st;
+  }
+  if (t->n_block2) {
+    int st = launch_lds_tier<K, MODE, 16>(g, t->d_block2, t->n_block2, t->max_block2_deg,
+                                          q + 3, out, work, "clique_lds_block_wide");
+    if (st != GPM_OK) return st;
   }
   if (t->n_wave) {
     const uint32_t chunk = t->wave_chunk;
@@ -448,6 +564,7 @@ int launch_tiers(gpm_graph* g, const RootTiers* t, unsigned long long* out,
     hipLaunchKernelGGL((clique_wave_kernel<K, MODE>), dim3(blocks), dim3(kBlock), 0, g->stream,
                        g->off, g->adj, t->d_wave, t->n_wave, chunk, q, out, work);
     GPM_HIP(hipGetLastError());
+    if (MODE == kCount) timing_mark(g, "clique_wave");
   }
   return GPM_OK;
 }
@@ -467,12 +584,15 @@ TierRequest request_from(const gpm_sched* s, uint32_t min_deg) {
   r.min_deg = min_deg;
   r.wave_cap = 64;
   r.block_cap = kBlockCap;
+  r.block2_cap = lds_wide_cap();
   if (s) {
     r.root_begin = s->root_begin;
     r.root_end = s->root_end;
     r.policy = s->policy;
     r.cost = s->root_cost;
     r.chunk = s->chunk;
+    r.root_list = s->root_list;
+    r.n_root_list = s->root_list ? s->n_root_list : 0;
     if (s->wave_cap && s->wave_cap < 64) r.wave_cap = s->wave_cap;
   }
   return r;
@@ -510,6 +630,15 @@ int gpm_clique(gpm_graph* g, int k, const gpm_sched* sched, uint64_t* out) {
   GPM_HIP(hipMemcpyAsync(&h, g->counters, sizeof(h), hipMemcpyDeviceToHost, g->stream));
   GPM_HIP(hipStreamSynchronize(g->stream));
   *out = h;
+  return GPM_OK;
+}
+
+int gpm_clique_tier_caps(uint32_t* wave_cap, uint32_t* lds_cap, uint32_t* wide_cap) {
+  clear_error();
+  GPM_CHECK_ARG(wave_cap && lds_cap && wide_cap, "null pointer");
+  *wave_cap = 64;
+  *lds_cap = kBlockCap;
+  *wide_cap = lds_wide_cap();
   return GPM_OK;
 }
 

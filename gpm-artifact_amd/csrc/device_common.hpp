@@ -31,6 +31,13 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
 
+// Declares a value the same in every lane (moves it to an SGPR).  Loop
+// bounds of loops that contain __syncthreads() go through this so that the
+// compiler emits them as scalar, wave-uniform loops.
+__device__ __forceinline__ uint32_t uniform(uint32_t v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
 __device__ __forceinline__ uint32_t wave_bcast(uint32_t v, int src) {
   return __shfl(v, src, kWave);
 }
@@ -92,9 +99,11 @@ __device__ __forceinline__ uint32_t hslot(uint32_t key, uint32_t mask) {
 
 // Insert `key` (unique) into an open-addressed LDS table; stores `val` in the
 // parallel value array.  The table must have room (load factor <= 1/2).
-__device__ __forceinline__ void lds_hash_insert(uint32_t* keys, uint16_t* vals,
+// Values are full 32-bit words: concurrent 16-bit stores from different waves
+// into the two halves of one LDS dword were seen to lose updates.
+__device__ __forceinline__ void lds_hash_insert(uint32_t* keys, uint32_t* vals,
                                                 uint32_t mask, uint32_t key,
-                                                uint16_t val) {
+                                                uint32_t val) {
   uint32_t h = hslot(key, mask);
   while (true) {
     uint32_t prev = atomicCAS(&keys[h], kEmpty, key);
@@ -108,7 +117,7 @@ __device__ __forceinline__ void lds_hash_insert(uint32_t* keys, uint16_t* vals,
 
 // Returns the stored value, or -1 when absent.
 __device__ __forceinline__ int lds_hash_find(const uint32_t* keys,
-                                             const uint16_t* vals,
+                                             const uint32_t* vals,
                                              uint32_t mask, uint32_t key) {
   uint32_t h = hslot(key, mask);
   while (true) {

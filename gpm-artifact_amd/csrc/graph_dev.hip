@@ -129,7 +129,17 @@ int zero_counters(gpm_graph* g, int count) {
 }
 
 int timing_begin(gpm_graph* g) {
+  g->n_marks = 0;
   GPM_HIP(hipEventRecord(g->ev0, g->stream));
+  return GPM_OK;
+}
+
+int timing_mark(gpm_graph* g, const char* name) {
+  if (g->n_marks >= gpm_graph::kMaxMarks) return GPM_OK;
+  hipEvent_t& e = g->mark_ev[g->n_marks];
+  if (!e) GPM_HIP(hipEventCreate(&e));
+  GPM_HIP(hipEventRecord(e, g->stream));
+  g->mark_name[g->n_marks++] = name;
   return GPM_OK;
 }
 
@@ -139,6 +149,11 @@ int timing_end(gpm_graph* g) {
   float ms = 0.f;
   GPM_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
   g->last_ms = ms;
+  for (int i = 0; i < g->n_marks; ++i) {
+    float mi = 0.f;
+    GPM_HIP(hipEventElapsedTime(&mi, i ? g->mark_ev[i - 1] : g->ev0, g->mark_ev[i]));
+    g->mark_ms[i] = mi;
+  }
   return GPM_OK;
 }
 
@@ -158,6 +173,8 @@ void release(gpm_graph* g) {
   hipFree(g->loff);
   hipFree(g->scratch);
   hipFree(g->counters);
+  for (auto& e : g->mark_ev)
+    if (e) (void)hipEventDestroy(e);
   if (g->ev0) hipEventDestroy(g->ev0);
   if (g->ev1) hipEventDestroy(g->ev1);
   if (g->stream) hipStreamDestroy(g->stream);
@@ -284,6 +301,19 @@ int gpm_last_kernel_ms(const gpm_graph* g, double* ms) {
   clear_error();
   GPM_CHECK_ARG(g && ms, "null pointer");
   *ms = g->last_ms;
+  return GPM_OK;
+}
+
+int gpm_last_kernel_breakdown(const gpm_graph* g, double* ms, const char** names,
+                              int capacity, int* count) {
+  clear_error();
+  GPM_CHECK_ARG(g && count, "null pointer");
+  const int n = std::min(capacity, g->n_marks);
+  for (int i = 0; i < n; ++i) {
+    if (ms) ms[i] = g->mark_ms[i];
+    if (names) names[i] = g->mark_name[i];
+  }
+  *count = g->n_marks;
   return GPM_OK;
 }
 

@@ -32,6 +32,12 @@ struct gpm_graph {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
+  // Per-kernel marks of the last call: an event after each launch.
+  static constexpr int kMaxMarks = 8;
+  hipEvent_t mark_ev[kMaxMarks] = {};
+  const char* mark_name[kMaxMarks] = {};
+  double mark_ms[kMaxMarks] = {};
+  int n_marks = 0;
   // Scratch: one growable device buffer plus a block of 64 counters.
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
@@ -53,6 +59,8 @@ int ensure_src(gpm_graph* g);
 int zero_counters(gpm_graph* g, int count);
 // Starts / stops event timing around the kernels of one mining call.
 int timing_begin(gpm_graph* g);
+// Records an event after the kernel just launched; `name` must be static.
+int timing_mark(gpm_graph* g, const char* name);
 int timing_end(gpm_graph* g);
 // Number of compute units of the handle's device.
 int device_cus(const gpm_graph* g);

@@ -79,8 +79,8 @@ __global__ __launch_bounds__(kBlock) void tc_vertex_wave_kernel(
     if (base >= n_roots) break;
     const uint32_t stop = min(base + chunk, n_roots);
     for (uint32_t r = base; r < stop; ++r) {
-      const uint32_t u = roots[r];
-      const uint32_t sb = off[u], d = off[u + 1] - sb;
+      const uint32_t u = uniform(roots[r]);
+      const uint32_t sb = uniform(off[u]), d = uniform(off[u + 1]) - sb;
       for (uint32_t i = lane; i < kTcWaveTable; i += kWave) keys[i] = kEmpty;
       __builtin_amdgcn_wave_barrier();
       for (uint32_t i = lane; i < d; i += kWave) {
@@ -148,11 +148,11 @@ __global__ __launch_bounds__(kBlock) void tc_vertex_block_kernel(
   while (true) {
     if (threadIdx.x == 0) s_root = atomicAdd(queue, 1u);
     __syncthreads();
-    const uint32_t r = s_root;
+    const uint32_t r = uniform(s_root);
     __syncthreads();
     if (r >= n_roots) break;
-    const uint32_t u = roots[r];
-    const uint32_t sb = off[u], d = off[u + 1] - sb;
+    const uint32_t u = uniform(roots[r]);
+    const uint32_t sb = uniform(off[u]), d = uniform(off[u + 1]) - sb;
     const uint32_t* S = adj + sb;
     if (table) {
       for (uint32_t i = threadIdx.x; i < table; i += kBlock) keys[i] = kEmpty;
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(kBlock) void tc_vertex_block_kernel(
       s_pre[threadIdx.x] = incl;
       s_beg[threadIdx.x] = b;
       __syncthreads();
-      const uint32_t total = s_pre[kBlock - 1];
+      const uint32_t total = uniform(s_pre[kBlock - 1]);
       for (uint32_t f = threadIdx.x; f < total; f += kBlock) {
         const uint32_t j = upper_index(s_pre, kBlock, f);
         const uint32_t excl = j ? s_pre[j - 1] : 0u;
@@ -233,6 +233,7 @@ extern "C" int gpm_tc(gpm_graph* g, int variant, uint64_t* out) {
       default: hipLaunchKernelGGL(tc_edge_kernel<64>, grid, block, 0, g->stream, g->off, g->adj, g->src, g->m, g->counters); break;
     }
     GPM_HIP(hipGetLastError());
+    timing_mark(g, "tc_edge");
   } else {
     const RootTiers* t = nullptr;
     if ((st = get_tc_tiers(g, kTcWaveCap, &t)) != GPM_OK) return st;
@@ -247,6 +248,7 @@ extern "C" int gpm_tc(gpm_graph* g, int variant, uint64_t* out) {
       hipLaunchKernelGGL(tc_vertex_wave_kernel, dim3(blocks), dim3(kBlock), 0, g->stream, g->off,
                          g->adj, t->d_wave, t->n_wave, chunk, q, g->counters);
       GPM_HIP(hipGetLastError());
+      timing_mark(g, "tc_vertex_wave");
     }
     if (t->n_block) {
       uint32_t table = 0;
@@ -259,6 +261,7 @@ extern "C" int gpm_tc(gpm_graph* g, int variant, uint64_t* out) {
                          table * sizeof(uint32_t), g->stream, g->off, g->adj, t->d_block,
                          t->n_block, table, q + 1, g->counters);
       GPM_HIP(hipGetLastError());
+      timing_mark(g, "tc_vertex_block");
     }
   }
   if ((st = timing_end(g)) != GPM_OK) return st;

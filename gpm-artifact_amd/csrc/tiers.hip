@@ -19,8 +19,11 @@ uint64_t checksum(const float* p, uint32_t n) {
 std::string key_of(const TierRequest& r, uint32_t n) {
   char buf[256];
   uint64_t cs = (r.policy == GPM_SCHED_LEARNED && r.cost) ? checksum(r.cost, n) : 0;
-  std::snprintf(buf, sizeof(buf), "%u/%u/%u/%u/%u/%d/%u/%g/%llx", r.min_deg, r.wave_cap,
-                r.block_cap, r.root_begin, r.root_end, r.policy, r.chunk, r.heavy_cost,
+  if (r.n_root_list)
+    cs ^= checksum(reinterpret_cast<const float*>(r.root_list), r.n_root_list) * 31 +
+          r.n_root_list;
+  std::snprintf(buf, sizeof(buf), "%u/%u/%u/%u/%u/%u/%d/%u/%g/%llx", r.min_deg, r.wave_cap,
+                r.block_cap, r.block2_cap, r.root_begin, r.root_end, r.policy, r.chunk, r.heavy_cost,
                 (unsigned long long)cs);
   return buf;
 }
@@ -45,10 +48,23 @@ int get_tiers(gpm_graph* g, const TierRequest& req, const RootTiers** out) {
   const uint32_t rb = req.root_begin;
   const uint32_t re = (req.root_end == 0 || req.root_end > g->n) ? g->n : req.root_end;
   if (rb > re) return fail(GPM_EINVAL, "root range [%u, %u) is empty or reversed", rb, re);
-  std::vector<uint32_t> wave, block, big;
-  uint32_t maxb = 0, maxg = 0;
+  std::vector<uint32_t> candidates;
+  if (req.n_root_list) {
+    std::vector<char> seen(g->n, 0);
+    for (uint32_t i = 0; i < req.n_root_list; ++i) {
+      const uint32_t v = req.root_list[i];
+      if (v >= g->n) return fail(GPM_EINVAL, "root_list[%u] = %u >= n", i, v);
+      if (seen[v]) return fail(GPM_EINVAL, "root %u listed twice", v);
+      seen[v] = 1;
+      candidates.push_back(v);
+    }
+  } else {
+    for (uint32_t v = rb; v < re; ++v) candidates.push_back(v);
+  }
+  std::vector<uint32_t> wave, block, block2, big;
+  uint32_t maxb = 0, maxb2 = 0, maxg = 0;
   const bool learned = req.policy == GPM_SCHED_LEARNED && req.cost;
-  for (uint32_t v = rb; v < re; ++v) {
+  for (const uint32_t v : candidates) {
     const uint32_t d = g->h_off[v + 1] - g->h_off[v];
     if (d < req.min_deg) continue;
     bool heavy = learned && req.heavy_cost > 0.f && req.cost[v] > req.heavy_cost;
@@ -57,6 +73,9 @@ int get_tiers(gpm_graph* g, const TierRequest& req, const RootTiers** out) {
     } else if (d <= req.block_cap) {
       block.push_back(v);
       maxb = std::max(maxb, d);
+    } else if (d <= req.block2_cap) {
+      block2.push_back(v);
+      maxb2 = std::max(maxb2, d);
     } else {
       big.push_back(v);
       maxg = std::max(maxg, d);
@@ -70,6 +89,7 @@ int get_tiers(gpm_graph* g, const TierRequest& req, const RootTiers** out) {
     };
     std::stable_sort(wave.begin(), wave.end(), by_cost);
     std::stable_sort(block.begin(), block.end(), by_cost);
+    std::stable_sort(block2.begin(), block2.end(), by_cost);
     std::stable_sort(big.begin(), big.end(), by_cost);
   }
   RootTiers t;
@@ -77,11 +97,14 @@ int get_tiers(gpm_graph* g, const TierRequest& req, const RootTiers** out) {
   t.n_block = (uint32_t)block.size();
   t.n_big = (uint32_t)big.size();
   t.max_block_deg = maxb;
+  t.n_block2 = (uint32_t)block2.size();
+  t.max_block2_deg = maxb2;
   t.max_big_deg = maxg;
   t.wave_chunk = req.chunk ? req.chunk : 8;
   int st;
   if ((st = upload(wave, &t.d_wave)) != GPM_OK) return st;
   if ((st = upload(block, &t.d_block)) != GPM_OK) return st;
+  if ((st = upload(block2, &t.d_block2)) != GPM_OK) return st;
   if ((st = upload(big, &t.d_big)) != GPM_OK) return st;
   if (hipMalloc(&t.queues, 4 * sizeof(unsigned int)) != hipSuccess)
     return fail(GPM_ENOMEM, "work queues");
@@ -91,8 +114,9 @@ int get_tiers(gpm_graph* g, const TierRequest& req, const RootTiers** out) {
   if (g->tier_cache.size() > 16) {
     RootTiers& old = g->tier_cache.front().second;
     hipFree(old.d_wave);
-    hipFree(old.d_block);
-    hipFree(old.d_big);
+    (void)hipFree(old.d_block);
+    (void)hipFree(old.d_block2);
+    (void)hipFree(old.d_big);
     hipFree(old.queues);
     g->tier_cache.erase(g->tier_cache.begin());
     *out = &g->tier_cache.back().second;
@@ -111,8 +135,9 @@ int get_tc_tiers(gpm_graph* g, uint32_t wave_cap, const RootTiers** out) {
 void free_tiers(gpm_graph* g) {
   for (auto& kv : g->tier_cache) {
     hipFree(kv.second.d_wave);
-    hipFree(kv.second.d_block);
-    hipFree(kv.second.d_big);
+    (void)hipFree(kv.second.d_block);
+    (void)hipFree(kv.second.d_block2);
+    (void)hipFree(kv.second.d_big);
     hipFree(kv.second.queues);
   }
   g->tier_cache.clear();

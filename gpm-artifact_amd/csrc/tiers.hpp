@@ -19,6 +19,9 @@ struct RootTiers {
   uint32_t* d_block = nullptr;  // roots served by one block, rows in LDS
   uint32_t n_block = 0;
   uint32_t max_block_deg = 0;
+  uint32_t* d_block2 = nullptr; // roots served by one block, wide LDS rows
+  uint32_t n_block2 = 0;
+  uint32_t max_block2_deg = 0;
   uint32_t* d_big = nullptr;    // roots served by one block, rows in HBM
   uint32_t n_big = 0;
   uint32_t max_big_deg = 0;
@@ -30,7 +33,10 @@ struct TierRequest {
   uint32_t min_deg = 1;       // roots below this out-degree do nothing
   uint32_t wave_cap = 64;     // max out-degree of a wave root
   uint32_t block_cap = 512;   // max out-degree of an LDS block root
+  uint32_t block2_cap = 512;  // max out-degree of a wide-LDS block root
   uint32_t root_begin = 0, root_end = 0;
+  const uint32_t* root_list = nullptr;  // explicit roots (overrides the range)
+  uint32_t n_root_list = 0;
   int policy = 0;             // gpm_sched_policy
   const float* cost = nullptr;  // per-vertex predicted cost (learned)
   uint32_t chunk = 0;         // 0 = automatic

@@ -42,7 +42,7 @@ class Sched(C.Structure):
     _fields_ = [("policy", C.c_int), ("chunk", C.c_uint32),
                 ("root_begin", C.c_uint32), ("root_end", C.c_uint32),
                 ("root_cost", _f32p), ("wave_cap", C.c_uint32),
-                ("reserved", C.c_uint32 * 3)]
+                ("n_root_list", C.c_uint32), ("root_list", _u32p)]
 
 
 class Query(C.Structure):
@@ -239,6 +239,14 @@ class DeviceGraph:
         _check(lib().gpm_last_kernel_ms(self._h, C.byref(ms)))
         return ms.value
 
+    def kernel_breakdown(self):
+        """[(kernel name, ms)] of the last call, in launch order."""
+        ms = (C.c_double * 8)()
+        names = (C.c_char_p * 8)()
+        cnt = C.c_int()
+        _check(lib().gpm_last_kernel_breakdown(self._h, ms, names, C.c_int(8), C.byref(cnt)))
+        return [(names[i].decode(), ms[i]) for i in range(min(8, cnt.value))]
+
     def tc(self, variant: int = TC_VERTEX) -> int:
         out = C.c_uint64()
         _check(lib().gpm_tc(self._h, C.c_int(variant), C.byref(out)))
@@ -246,13 +254,18 @@ class DeviceGraph:
 
     def clique(self, k: int, policy: int = SCHED_DEGREE, chunk: int = 0,
                root_begin: int = 0, root_end: int = 0,
-               cost: Optional[np.ndarray] = None, wave_cap: int = 0) -> int:
+               cost: Optional[np.ndarray] = None, wave_cap: int = 0,
+               roots: Optional[np.ndarray] = None) -> int:
         s = Sched()
         s.policy, s.chunk = policy, chunk
         s.root_begin, s.root_end, s.wave_cap = root_begin, root_end, wave_cap
         if cost is not None:
             self._cost = np.ascontiguousarray(cost, dtype=np.float32)
             s.root_cost = _ptr(self._cost, C.c_float)
+        if roots is not None:
+            self._roots = _u32(roots)
+            s.n_root_list = len(self._roots)
+            s.root_list = _ptr(self._roots)
         out = C.c_uint64()
         _check(lib().gpm_clique(self._h, C.c_int(k), C.byref(s), C.byref(out)))
         return out.value
@@ -293,6 +306,13 @@ class DeviceGraph:
         _check(lib().gpm_match_batch(self._h, arr, C.c_int(len(queries)),
                                      C.byref(t), _ptr(out, C.c_uint64)))
         return out
+
+
+def clique_tier_caps():
+    """(wave_cap, lds_cap, wide_cap) out-degree limits of the clique tiers."""
+    a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    _check(lib().gpm_clique_tier_caps(C.byref(a), C.byref(b), C.byref(c)))
+    return a.value, b.value, c.value
 
 
 def vertex_features(sym: DeviceGraph, dag: Optional[DeviceGraph]) -> np.ndarray:

@@ -141,6 +141,13 @@ int gpm_graph_orient(gpm_graph* sym, gpm_graph** dag);
 
 /* Event-timed duration (ms) of the kernels of the last mining call on g. */
 int gpm_last_kernel_ms(const gpm_graph* g, double* ms);
+/*
+ * Per-kernel split of the last mining call: ms[i] is the event-timed span
+ * ending with the i-th main kernel (names[i] names it); *count receives the
+ * number of kernels, of which min(capacity, *count) are written.
+ */
+int gpm_last_kernel_breakdown(const gpm_graph* g, double* ms, const char** names,
+                              int capacity, int* count);
 
 /* ------------------------------------------------------------ scheduling */
 
@@ -156,7 +163,8 @@ typedef struct gpm_sched {
   uint32_t root_end;       /*   root_end == 0 means "all vertices"            */
   const float* root_cost;  /* LEARNED: predicted cost per vertex (n floats)   */
   uint32_t wave_cap;       /* local-set size served by one wave, 0 = 64       */
-  uint32_t reserved[3];
+  uint32_t n_root_list;    /* > 0: only the roots in root_list (host array),  */
+  const uint32_t* root_list; /*   each vertex at most once, ids < n           */
 } gpm_sched;
 
 /* ------------------------------------------------------------- mining */
@@ -177,6 +185,14 @@ int gpm_tc(gpm_graph* g, int variant, uint64_t* out);
  * (src/CFSolver.cuh:99-157, which supports k = 4, 5).  sched may be NULL.
  */
 int gpm_clique(gpm_graph* g, int k, const gpm_sched* sched, uint64_t* out);
+
+/*
+ * Out-degree limits of the k-clique kernels on this device: roots with
+ * out-degree <= *wave_cap run on one wave (rows in registers), <= *lds_cap
+ * on one block with LDS rows, <= *wide_cap on one block with wide LDS rows,
+ * above that on one block with rows in HBM.
+ */
+int gpm_clique_tier_caps(uint32_t* wave_cap, uint32_t* lds_cap, uint32_t* wide_cap);
 
 /*
  * Per-root work log for the cost model: work[v] = number of adjacency

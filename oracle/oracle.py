@@ -27,8 +27,8 @@ def lib() -> C.CDLL:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} not built (make -C oracle)")
         L = C.CDLL(LIB_PATH)
-        for name in ("orc_tc", "orc_clique", "orc_plan_bytes", "orc_pattern",
-                     "orc_ceci"):
+        for name in ("orc_tc", "orc_clique", "orc_clique_list", "orc_plan_bytes",
+                     "orc_pattern", "orc_ceci"):
             getattr(L, name).restype = C.c_uint64
         L.orc_num_threads.restype = C.c_int
         _lib = L
@@ -61,6 +61,12 @@ def clique(off, adj, k: int, root_begin: int = 0, root_end: int = 0) -> int:
     return lib().orc_clique(C.c_uint32(len(off) - 1), _p(off), _p(adj),
                             C.c_int(k), C.c_uint32(root_begin),
                             C.c_uint32(root_end))
+
+
+def clique_list(off, adj, k: int, roots) -> int:
+    off, adj, roots = _u32(off), _u32(adj), _u32(roots)
+    return lib().orc_clique_list(C.c_uint32(len(off) - 1), _p(off), _p(adj),
+                                 C.c_int(k), _p(roots), C.c_uint32(len(roots)))
 
 
 def plan_bytes(off, adj, k: int) -> int:
