@@ -33,7 +33,10 @@ __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
 
 // Declares a value the same in every lane (moves it to an SGPR).  Loop
 // bounds of loops that contain __syncthreads() go through this so that the
-// compiler emits them as scalar, wave-uniform loops.
+// compiler emits them as scalar, wave-uniform loops.  Bounds loaded from
+// LDS or global memory without it are treated as possibly divergent; the
+// block kernels then gave nondeterministic counts once two blocks shared a
+// compute unit, and exact counts with it.
 __device__ __forceinline__ uint32_t uniform(uint32_t v) {
   return __builtin_amdgcn_readfirstlane(v);
 }
@@ -99,8 +102,6 @@ __device__ __forceinline__ uint32_t hslot(uint32_t key, uint32_t mask) {
 
 // Insert `key` (unique) into an open-addressed LDS table; stores `val` in the
 // parallel value array.  The table must have room (load factor <= 1/2).
-// Values are full 32-bit words: concurrent 16-bit stores from different waves
-// into the two halves of one LDS dword were seen to lose updates.
 __device__ __forceinline__ void lds_hash_insert(uint32_t* keys, uint32_t* vals,
                                                 uint32_t mask, uint32_t key,
                                                 uint32_t val) {
