@@ -164,6 +164,8 @@ namespace {
 void release(gpm_graph* g) {
   if (!g) return;
   if (g->stream) hipStreamSynchronize(g->stream);
+  if (g->dag) gpm_graph_destroy(g->dag);
+  hipFree(g->rank);
   free_tiers(g);
   hipFree(g->off);
   hipFree(g->adj);

@@ -46,6 +46,10 @@ struct gpm_graph {
   std::vector<std::pair<std::string, gpm::RootTiers>> tier_cache;
   // Cached algorithmic-byte totals per k (0 = not computed yet).
   uint64_t plan_bytes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // Symmetric graphs: degree rank (position in (degree, id) order) and the
+  // degree-oriented DAG, both built on first use by the motif counters.
+  uint32_t* rank = nullptr;
+  gpm_graph* dag = nullptr;
 };
 
 namespace gpm {

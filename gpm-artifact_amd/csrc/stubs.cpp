@@ -6,10 +6,6 @@ using gpm::fail;
 
 extern "C" {
 
-int gpm_pattern(gpm_graph*, int, int, uint64_t*, uint64_t*) {
-  return fail(GPM_EUNSUPPORTED, "gpm_pattern: not built yet");
-}
-
 int gpm_match_batch(gpm_graph*, const gpm_query*, int, const gpm_tuning*, uint64_t*) {
   return fail(GPM_EUNSUPPORTED, "gpm_match_batch: not built yet");
 }

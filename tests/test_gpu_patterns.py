@@ -1,0 +1,99 @@
+"""GPU parity: fixed patterns P1..P8 and Motif4 (both formulations) vs the
+census-pinned oracle, and the C++ drivers over the Engine shim."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import graphs
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "gpm-artifact_amd", "bin")
+
+
+def cases(gpm):
+    return [
+        ("er_small", graphs.er(40, 160, 1)),
+        ("er_dense", graphs.er(30, 200, 2)),
+        ("wheel", graphs.wheel(40)),
+        ("k12", graphs.complete(12)),
+        ("rmat9", gpm.rmat_graph(9, 8, 3)),
+        ("rmat11", gpm.rmat_graph(11, 16, 4)),
+    ]
+
+
+@pytest.fixture(scope="module")
+def graph_cases(gpm):
+    return cases(gpm)
+
+
+def test_motif4_both_formulations(gpm, orc, graph_cases):
+    for name, g in graph_cases:
+        want = orc.motif4(g.offsets, g.adj)
+        with gpm.DeviceGraph(g, oriented=False) as dg:
+            direct, s_direct = dg.pattern("Motif4", gpm.FORM_G2MINER)
+            ident, s_ident = dg.pattern("Motif4", gpm.FORM_PATTERN)
+        assert direct == want, name
+        assert ident == want, name
+        assert s_direct == want[0]
+        assert s_ident == (want[1] - want[2]) % (1 << 64)
+
+
+@pytest.mark.parametrize("pid", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_patterns_direct(gpm, orc, graph_cases, pid):
+    for name, g in graph_cases:
+        want = orc.pattern(g.offsets, g.adj, pid)
+        with gpm.DeviceGraph(g, oriented=False) as dg:
+            out, scalar = dg.pattern(f"P{pid}", gpm.FORM_G2MINER)
+            out2, scalar2 = dg.pattern(f"P{pid}", gpm.FORM_PATTERN)
+        assert out[0] == want and scalar == want, (name, pid)
+        assert scalar2 == want, (name, pid)
+
+
+def test_motif4_identities_larger(gpm, orc):
+    g = gpm.rmat_graph(14, 16, 7)
+    want = orc.motif4(g.offsets, g.adj)
+    with gpm.DeviceGraph(g, oriented=False) as dg:
+        assert dg.pattern("Motif4", gpm.FORM_PATTERN)[0] == want
+        assert dg.pattern("Motif4", gpm.FORM_G2MINER)[0] == want
+
+
+def test_pattern_errors(gpm):
+    g = graphs.complete(6)
+    with gpm.DeviceGraph(gpm.orient(g), oriented=True) as dg:
+        with pytest.raises(gpm.GpmError):
+            dg.pattern("P1")  # needs a symmetric graph
+
+
+def _write_mtx(path, g):
+    with open(path, "w") as f:
+        f.write("%%MatrixMarket matrix coordinate pattern general\n")
+        f.write(f"{g.n} {g.n} {g.m}\n")
+        for u in range(g.n):
+            for v in g.adj[g.offsets[u]:g.offsets[u + 1]]:
+                f.write(f"{u + 1} {int(v) + 1}\n")
+
+
+def test_drivers(gpm, orc, tmp_path):
+    g = gpm.rmat_graph(10, 8, 11)
+    path = str(tmp_path / "g.mtx")
+    _write_mtx(path, g)
+    d = gpm.orient(g)
+    tri = orc.tc(d.offsets, d.adj)
+    for mode in ("v_centric", "e_centric"):
+        out = subprocess.run([os.path.join(BIN, "test_trianglecounting"), path, mode],
+                             capture_output=True, text=True, check=True).stdout
+        assert f"Pattern 0: {tri}" in out
+    out = subprocess.run([os.path.join(BIN, "test_clique"), path, "4"],
+                         capture_output=True, text=True, check=True).stdout
+    assert f"Result: {orc.clique(d.offsets, d.adj, 4)}" in out
+    m4 = orc.motif4(g.offsets, g.adj)
+    out = subprocess.run([os.path.join(BIN, "test_G2Miner"), path, "Motif4"],
+                         capture_output=True, text=True, check=True).stdout
+    assert f"Result: {m4[0]}" in out and f"Result5: {m4[5]}" in out
+    out = subprocess.run([os.path.join(BIN, "test_G2Miner"), path, "P3", "patternenum"],
+                         capture_output=True, text=True, check=True).stdout
+    assert f"Result: {orc.pattern(g.offsets, g.adj, 3)}" in out
