@@ -164,6 +164,7 @@ namespace {
 void release(gpm_graph* g) {
   if (!g) return;
   if (g->stream) hipStreamSynchronize(g->stream);
+  free_match_cache(g);
   if (g->dag) gpm_graph_destroy(g->dag);
   hipFree(g->rank);
   free_tiers(g);
@@ -173,6 +174,7 @@ void release(gpm_graph* g) {
   hipFree(g->labels);
   hipFree(g->ladj);
   hipFree(g->loff);
+  hipFree(g->lverts);
   hipFree(g->scratch);
   hipFree(g->counters);
   for (auto& e : g->mark_ev)
@@ -194,6 +196,17 @@ int build_label_index(gpm_graph* g, const uint32_t* offsets, const uint32_t* adj
     return fail(GPM_ENOMEM, "device label index");
   GPM_HIP(hipMemcpyAsync(g->ladj, ladj.data(), g->m * 4, hipMemcpyHostToDevice, g->stream));
   GPM_HIP(hipMemcpyAsync(g->loff, loff.data(), loff.size() * 4, hipMemcpyHostToDevice,
+                         g->stream));
+  // Vertices grouped by label (counting sort, ids ascending inside a label).
+  const uint32_t L = g->n_labels;
+  g->h_lvoff.assign(L + 1, 0);
+  for (uint32_t v = 0; v < g->n; ++v) g->h_lvoff[labels[v] + 1]++;
+  for (uint32_t l = 0; l < L; ++l) g->h_lvoff[l + 1] += g->h_lvoff[l];
+  std::vector<uint32_t> lverts(std::max(g->n, 1u)), cur(g->h_lvoff.begin(), g->h_lvoff.end() - 1);
+  for (uint32_t v = 0; v < g->n; ++v) lverts[cur[labels[v]]++] = v;
+  if (hipMalloc(&g->lverts, lverts.size() * 4) != hipSuccess)
+    return fail(GPM_ENOMEM, "device label vertex lists");
+  GPM_HIP(hipMemcpyAsync(g->lverts, lverts.data(), (size_t)g->n * 4, hipMemcpyHostToDevice,
                          g->stream));
   GPM_HIP(hipStreamSynchronize(g->stream));
   return GPM_OK;

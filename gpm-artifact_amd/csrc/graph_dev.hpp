@@ -27,6 +27,9 @@ struct gpm_graph {
   uint32_t* labels = nullptr;
   uint32_t* ladj = nullptr;
   uint32_t* loff = nullptr;
+  // Vertices grouped by label: lverts[h_lvoff[l] .. h_lvoff[l+1]) have label l.
+  uint32_t* lverts = nullptr;
+  std::vector<uint32_t> h_lvoff;
   std::vector<uint32_t> h_off;
   std::vector<uint32_t> h_labels;
   hipStream_t stream = nullptr;
@@ -50,6 +53,8 @@ struct gpm_graph {
   // degree-oriented DAG, both built on first use by the motif counters.
   uint32_t* rank = nullptr;
   gpm_graph* dag = nullptr;
+  // Last prepared matching batch (match.hip), replayed on identical calls.
+  void* match_cache = nullptr;
 };
 
 namespace gpm {
@@ -68,5 +73,7 @@ int timing_mark(gpm_graph* g, const char* name);
 int timing_end(gpm_graph* g);
 // Number of compute units of the handle's device.
 int device_cus(const gpm_graph* g);
+// Releases the cached matching batch of g (match.hip).
+void free_match_cache(gpm_graph* g);
 
 }  // namespace gpm

@@ -1,0 +1,663 @@
+// Labeled subgraph matching, many queries per launch (gpm_match_batch).
+//
+// Counts, for every query, the injective label- and edge-preserving maps
+// into the data graph (non-induced, automorphisms counted): the number the
+// reference's CPU CECI matcher prints as "#Embeddings"
+// (sub_ceci/myceci.cpp:540-668).  The GPU path is new work; its pieces:
+//
+//  * QueryingOrder (host).  Root = the query vertex minimising
+//    |V(label)| / degree (the CECI start rule, myceci.cpp:15-31).  The plain
+//    order is the BFS order from the root (myceci.cpp:65-103); the selective
+//    order instead adds, at each step, the vertex with the most already-
+//    placed neighbours (ties: rarer label), so that candidate sets are
+//    intersections of several lists as early as possible.
+//  * Candidate filter at every depth: right label, enough degree and, per
+//    neighbour label, enough neighbours of that label (the NLF filter of
+//    myceci.cpp:120-169), read in O(1) per label from the label-partitioned
+//    adjacency.  Without it the search explores far more dead partial
+//    embeddings (on the reference's HPRD queries, the largest root needed
+//    1.8M serial steps without it and 60K with it).
+//  * Enumeration (device): one wave per task (a chunk of root candidates of
+//    one query), tasks pulled from one queue by a persistent grid.  The DFS
+//    keeps no candidate lists: at depth d it streams the label-l slice of the
+//    shortest backward neighbour's list 64 entries at a time, keeps a 64-bit
+//    ballot of the entries that pass (degree, injectivity, membership in the
+//    other backward neighbours' label slices by binary search) and descends
+//    into the set bits one by one; the last depth only counts ballots.  All
+//    DFS state is wave-uniform and lives in LDS.
+//  * Autotuner: a probe launch over a sample of every query's roots measures
+//    the cycles one root costs; each query's chunk (roots per task) is set so
+//    a task costs about the same, and queries are laid out heaviest first.
+//  * The tuned launch (counter reset + enumeration) is captured once into a
+//    hipGraph and replayed.
+#include <algorithm>
+#include <numeric>
+#include <queue>
+#include <vector>
+
+#include "graph_dev.hpp"
+
+using namespace gpm;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kMaxQ = 16;
+
+struct QPlan {
+  uint32_t n;               // query vertices
+  uint32_t label[kMaxQ];    // label per depth
+  uint32_t qdeg[kMaxQ];     // query degree per depth
+  uint32_t bn[kMaxQ];       // backward-neighbour depths (bitmask) per depth
+  uint32_t same[kMaxQ];     // earlier depths with the same label (bitmask)
+  uint32_t root_off;        // first root of this query in the root array
+  uint32_t n_roots;
+  uint32_t chunk;           // roots per task
+  uint32_t task_off;        // first task of this query
+  uint32_t qid;             // index of the query in the caller's array
+  // Neighbour-label filter per depth: a data vertex matched at depth d needs
+  // at least nlf_count[d][k] neighbours of label nlf_label[d][k].
+  uint32_t nlf_n[kMaxQ];
+  uint32_t nlf_label[kMaxQ][kMaxQ];
+  uint32_t nlf_count[kMaxQ][kMaxQ];
+};
+
+struct DevGraph {
+  const uint32_t* off;
+  const uint32_t* ladj;
+  const uint32_t* loff;
+  uint32_t L;
+};
+
+__device__ __forceinline__ void label_slice(const DevGraph& g, uint32_t w, uint32_t l,
+                                            uint32_t* lo, uint32_t* len) {
+  const uint32_t* p = g.loff + (size_t)w * (g.L + 1) + l;
+  const uint32_t a = p[0], b = p[1];
+  *lo = a;
+  *len = b - a;
+}
+
+// Root candidates: qplan p, scan vertices of label p.label[0].
+__global__ __launch_bounds__(kBlock) void roots_kernel(DevGraph g, const uint32_t* __restrict__ lverts,
+                                                       const uint32_t* __restrict__ lvoff,
+                                                       const QPlan* __restrict__ plans,
+                                                       uint32_t* __restrict__ roots,
+                                                       unsigned int* __restrict__ counts) {
+  const QPlan& p = plans[blockIdx.y];
+  const uint32_t l0 = p.label[0];
+  const uint32_t b = lvoff[l0], e = lvoff[l0 + 1];
+  for (uint32_t i = b + blockIdx.x * blockDim.x + threadIdx.x; i < e;
+       i += gridDim.x * blockDim.x) {
+    const uint32_t v = lverts[i];
+    bool ok = g.off[v + 1] - g.off[v] >= p.qdeg[0];
+    for (uint32_t k = 0; ok && k < p.nlf_n[0]; ++k) {
+      uint32_t lo, len;
+      label_slice(g, v, p.nlf_label[0][k], &lo, &len);
+      ok = len >= p.nlf_count[0][k];
+    }
+    if (ok) {
+      const uint32_t at = atomicAdd(&counts[blockIdx.y], 1u);
+      roots[p.root_off + at] = v;
+    }
+  }
+}
+
+struct WaveState {
+  uint32_t emb[kMaxQ];
+  uint32_t cur[kMaxQ];   // next slice position to load
+  uint32_t base[kMaxQ];  // slice position of the current window
+  uint32_t sb[kMaxQ];    // slice start (in ladj)
+  uint32_t sl[kMaxQ];    // slice length
+  uint32_t anc[kMaxQ];   // anchor depth (its slice is the one streamed)
+  unsigned long long mask[kMaxQ];
+};
+
+// Sets up depth d: picks the backward neighbour with the shortest label slice.
+__device__ __forceinline__ void init_depth(const DevGraph& g, const QPlan& p, WaveState& s,
+                                           uint32_t d) {
+  uint32_t bn = p.bn[d], best = 0xFFFFFFFFu, blo = 0, bdep = 0;
+  while (bn) {
+    const uint32_t b = __builtin_ctz(bn);
+    bn &= bn - 1;
+    uint32_t lo, len;
+    label_slice(g, s.emb[b], p.label[d], &lo, &len);
+    if (len < best) {
+      best = len;
+      blo = lo;
+      bdep = b;
+    }
+  }
+  s.sb[d] = blo;
+  s.sl[d] = best;
+  s.anc[d] = bdep;
+  s.cur[d] = 0;
+  s.mask[d] = 0ull;
+}
+
+// Does slice entry x pass the filters of depth d?
+__device__ __forceinline__ bool passes(const DevGraph& g, const QPlan& p, const WaveState& s,
+                                       uint32_t d, uint32_t x) {
+  if (g.off[x + 1] - g.off[x] < p.qdeg[d]) return false;
+  for (uint32_t k = 0; k < p.nlf_n[d]; ++k) {
+    uint32_t lo, len;
+    label_slice(g, x, p.nlf_label[d][k], &lo, &len);
+    if (len < p.nlf_count[d][k]) return false;
+  }
+  uint32_t same = p.same[d];
+  while (same) {
+    const uint32_t j = __builtin_ctz(same);
+    same &= same - 1;
+    if (s.emb[j] == x) return false;
+  }
+  uint32_t bn = p.bn[d] & ~(1u << s.anc[d]);
+  while (bn) {
+    const uint32_t b = __builtin_ctz(bn);
+    bn &= bn - 1;
+    uint32_t lo, len;
+    label_slice(g, s.emb[b], p.label[d], &lo, &len);
+    if (!sorted_contains(g.ladj + lo, len, x)) return false;
+  }
+  return true;
+}
+
+// Counts the embeddings of plan p rooted at v0.
+__device__ uint64_t count_root(const DevGraph& g, const QPlan& p, WaveState& s, uint32_t v0) {
+  const uint32_t lane = lane_id();
+  const uint32_t n = p.n;
+  if (n == 1) return 1;
+  uint64_t total = 0;
+  s.emb[0] = v0;
+  init_depth(g, p, s, 1);
+  int d = 1;
+  while (d >= 1) {
+    const uint32_t dd = (uint32_t)d;
+    if (s.mask[dd] == 0ull) {
+      const uint32_t cur = s.cur[dd];
+      if (cur >= s.sl[dd]) {
+        --d;
+        continue;
+      }
+      const uint32_t i = cur + lane;
+      const bool ok = i < s.sl[dd] && passes(g, p, s, dd, g.ladj[s.sb[dd] + i]);
+      const uint64_t m = wave_ballot(ok);
+      s.cur[dd] = cur + kWave;
+      if (dd == n - 1) {
+        total += (uint64_t)__popcll(m);
+      } else {
+        s.mask[dd] = m;
+        s.base[dd] = cur;
+      }
+      continue;
+    }
+    const unsigned long long m = s.mask[dd];
+    const uint32_t j = (uint32_t)__builtin_ctzll(m);
+    s.mask[dd] = m & (m - 1);
+    s.emb[dd] = uniform(g.ladj[s.sb[dd] + s.base[dd] + j]);
+    init_depth(g, p, s, dd + 1);
+    ++d;
+  }
+  return total;
+}
+
+__device__ __forceinline__ uint32_t find_plan(const QPlan* plans, uint32_t nq, uint32_t t) {
+  uint32_t lo = 0, hi = nq;  // last plan with task_off <= t
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (plans[mid].task_off <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Persistent enumeration.  PROBE: tasks are sampled roots (one each); the
+// cycles spent are added to cycles[plan] instead of counting.
+template <bool PROBE>
+__global__ __launch_bounds__(kBlock) void match_kernel(
+    DevGraph g, const QPlan* __restrict__ plans, uint32_t nq, uint32_t n_tasks,
+    const uint32_t* __restrict__ roots, const uint32_t* __restrict__ sample_stride,
+    unsigned int* __restrict__ queue, unsigned long long* __restrict__ counts,
+    unsigned long long* __restrict__ cycles) {
+  __shared__ WaveState s_state[kWavesPerBlock];
+  WaveState& s = s_state[threadIdx.x / kWave];
+  const uint32_t lane = lane_id();
+  while (true) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(queue, 1u);
+    t = uniform(wave_bcast(t, 0));
+    if (t >= n_tasks) break;
+    const uint32_t pi = uniform(find_plan(plans, nq, t));
+    const QPlan& p = plans[pi];
+    const uint32_t k = t - p.task_off;
+    uint64_t c = 0;
+    if (PROBE) {
+      const uint32_t r = k * sample_stride[pi];
+      const long long t0 = clock64();
+      c = count_root(g, p, s, uniform(roots[p.root_off + r]));
+      const long long t1 = clock64();
+      if (lane == 0) atomicAdd(&cycles[pi], (unsigned long long)(t1 - t0) + (c & 0));
+    } else {
+      const uint32_t r0 = k * p.chunk;
+      const uint32_t r1 = min(r0 + p.chunk, p.n_roots);
+      for (uint32_t r = r0; r < r1; ++r) c += count_root(g, p, s, uniform(roots[p.root_off + r]));
+      if (lane == 0 && c) atomicAdd(&counts[p.qid], (unsigned long long)c);
+    }
+  }
+}
+
+__global__ void zero_kernel(unsigned long long* a, int na, unsigned int* b, int nb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < na) a[i] = 0ull;
+  if (i < nb) b[i] = 0u;
+}
+
+struct HostQuery {
+  uint32_t n;
+  std::vector<std::vector<uint32_t>> adj;
+  std::vector<uint32_t> label;
+};
+
+// Builds the plan of one query; returns false when it cannot match at all
+// (a label the data graph does not have).
+bool make_plan(const gpm_graph* g, const HostQuery& q, int policy, QPlan* p) {
+  const uint32_t n = q.n;
+  for (uint32_t u = 0; u < n; ++u)
+    if (q.label[u] >= g->n_labels) return false;
+  auto lab_count = [&](uint32_t l) { return (double)(g->h_lvoff[l + 1] - g->h_lvoff[l]); };
+  uint32_t start = 0;
+  double best = 1e300;
+  for (uint32_t u = 0; u < n; ++u) {
+    const double deg = std::max<double>(1.0, (double)q.adj[u].size());
+    const double score = lab_count(q.label[u]) / deg;
+    if (score < best) {
+      best = score;
+      start = u;
+    }
+  }
+  std::vector<uint32_t> order;
+  std::vector<char> placed(n, 0);
+  if (policy == GPM_SCHED_LEARNED) {
+    order.push_back(start);
+    placed[start] = 1;
+    while (order.size() < n) {
+      int bu = -1, bconn = -1;
+      double brare = 1e300;
+      for (uint32_t u = 0; u < n; ++u) {
+        if (placed[u]) continue;
+        int conn = 0;
+        for (uint32_t w : q.adj[u]) conn += placed[w];
+        if (conn == 0) continue;
+        const double rare = lab_count(q.label[u]) / std::max<double>(1.0, q.adj[u].size());
+        if (conn > bconn || (conn == bconn && rare < brare)) {
+          bconn = conn;
+          brare = rare;
+          bu = (int)u;
+        }
+      }
+      if (bu < 0) return false;  // disconnected (checked by the caller)
+      placed[bu] = 1;
+      order.push_back((uint32_t)bu);
+    }
+  } else {
+    std::queue<uint32_t> bq;
+    bq.push(start);
+    placed[start] = 1;
+    while (!bq.empty()) {
+      const uint32_t u = bq.front();
+      bq.pop();
+      order.push_back(u);
+      for (uint32_t w : q.adj[u])
+        if (!placed[w]) {
+          placed[w] = 1;
+          bq.push(w);
+        }
+    }
+  }
+  std::vector<uint32_t> pos(n);
+  for (uint32_t i = 0; i < n; ++i) pos[order[i]] = i;
+  std::memset(p, 0, sizeof(QPlan));
+  p->n = n;
+  for (uint32_t d = 0; d < n; ++d) {
+    const uint32_t u = order[d];
+    p->label[d] = q.label[u];
+    p->qdeg[d] = (uint32_t)q.adj[u].size();
+    for (uint32_t w : q.adj[u])
+      if (pos[w] < d) p->bn[d] |= 1u << pos[w];
+    for (uint32_t j = 0; j < d; ++j)
+      if (q.label[order[j]] == q.label[u]) p->same[d] |= 1u << j;
+  }
+  // Neighbour-label requirements of every depth.
+  for (uint32_t d = 0; d < n; ++d) {
+    std::vector<std::pair<uint32_t, uint32_t>> need;
+    for (uint32_t w : q.adj[order[d]]) {
+      bool found = false;
+      for (auto& kv : need)
+        if (kv.first == q.label[w]) {
+          kv.second++;
+          found = true;
+        }
+      if (!found) need.emplace_back(q.label[w], 1u);
+    }
+    p->nlf_n[d] = (uint32_t)need.size();
+    for (size_t i = 0; i < need.size(); ++i) {
+      p->nlf_label[d][i] = need[i].first;
+      p->nlf_count[d][i] = need[i].second;
+    }
+  }
+  return true;
+}
+
+bool connected(const HostQuery& q) {
+  if (q.n == 0) return false;
+  std::vector<char> seen(q.n, 0);
+  std::vector<uint32_t> st{0};
+  seen[0] = 1;
+  uint32_t cnt = 1;
+  while (!st.empty()) {
+    const uint32_t u = st.back();
+    st.pop_back();
+    for (uint32_t w : q.adj[u])
+      if (!seen[w]) {
+        seen[w] = 1;
+        ++cnt;
+        st.push_back(w);
+      }
+  }
+  return cnt == q.n;
+}
+
+// A prepared batch: plans laid out for the persistent kernel, root
+// candidates, counters, and (when asked for) the instantiated hipGraph of
+// the counter reset + enumeration launch.  The last batch is kept on the
+// graph handle, so calling again with the same queries and tuning replays
+// the graph without planning, filtering or tuning again.
+struct MatchBatch {
+  uint64_t key = 0;
+  void* mem = nullptr;
+  QPlan* d_plans = nullptr;
+  uint32_t* d_roots = nullptr;
+  unsigned long long* d_counts = nullptr;
+  unsigned long long* d_cycles = nullptr;
+  unsigned int* d_rootcnt = nullptr;
+  unsigned int* d_stride = nullptr;
+  unsigned int* d_queue = nullptr;
+  uint32_t nq = 0, n_tasks = 0;
+  int n_queries = 0;
+  unsigned grid = 0;
+  DevGraph dg{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+void free_batch(void* p) {
+  MatchBatch* b = static_cast<MatchBatch*>(p);
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  (void)hipFree(b->mem);
+  delete b;
+}
+
+uint64_t batch_key(const gpm_query* queries, int n_queries, const gpm_tuning& t) {
+  uint64_t h = 1469598103934665603ull;
+  auto mixin = [&](uint64_t x) { h = (h ^ x) * 1099511628211ull; };
+  mixin((uint64_t)n_queries);
+  mixin((uint64_t)t.autotune);
+  mixin((uint64_t)t.policy);
+  mixin((uint64_t)t.chunk);
+  mixin((uint64_t)t.use_graph);
+  for (int i = 0; i < n_queries; ++i) {
+    mixin(queries[i].n);
+    mixin(queries[i].n_edges);
+    for (uint32_t e = 0; e < 2 * queries[i].n_edges; ++e) mixin(queries[i].edges[e]);
+    for (uint32_t v = 0; v < queries[i].n; ++v) mixin(queries[i].labels[v]);
+  }
+  return h;
+}
+
+// Enqueues counter reset + enumeration of a prepared batch.
+int launch_batch(gpm_graph* g, MatchBatch* b) {
+  if (b->exec) {
+    GPM_HIP(hipGraphLaunch(b->exec, g->stream));
+    return GPM_OK;
+  }
+  hipLaunchKernelGGL(zero_kernel, dim3((b->n_queries + 64) / 64 + 1), dim3(64), 0, g->stream,
+                     b->d_counts, b->n_queries, b->d_queue, 4);
+  hipLaunchKernelGGL(match_kernel<false>, dim3(b->grid), dim3(kBlock), 0, g->stream, b->dg,
+                     b->d_plans, b->nq, b->n_tasks, b->d_roots, b->d_stride, b->d_queue,
+                     b->d_counts, b->d_cycles);
+  GPM_HIP(hipGetLastError());
+  return GPM_OK;
+}
+
+// Builds the hipGraph of launch_batch explicitly: two kernel nodes, the
+// enumeration depending on the reset (no stream capture, no memset node).
+int build_graph(MatchBatch* b) {
+  GPM_HIP(hipGraphCreate(&b->graph, 0));
+  int n_queries = b->n_queries, four = 4;
+  void* zargs[] = {&b->d_counts, &n_queries, &b->d_queue, &four};
+  hipKernelNodeParams zp{};
+  zp.func = reinterpret_cast<void*>(&zero_kernel);
+  zp.gridDim = dim3((b->n_queries + 64) / 64 + 1);
+  zp.blockDim = dim3(64);
+  zp.sharedMemBytes = 0;
+  zp.kernelParams = zargs;
+  zp.extra = nullptr;
+  hipGraphNode_t zn = nullptr, mn = nullptr;
+  GPM_HIP(hipGraphAddKernelNode(&zn, b->graph, nullptr, 0, &zp));
+  void* margs[] = {&b->dg,      &b->d_plans, &b->nq,       &b->n_tasks, &b->d_roots,
+                   &b->d_stride, &b->d_queue, &b->d_counts, &b->d_cycles};
+  hipKernelNodeParams mp{};
+  mp.func = reinterpret_cast<void*>(&match_kernel<false>);
+  mp.gridDim = dim3(b->grid);
+  mp.blockDim = dim3(kBlock);
+  mp.sharedMemBytes = 0;
+  mp.kernelParams = margs;
+  mp.extra = nullptr;
+  GPM_HIP(hipGraphAddKernelNode(&mn, b->graph, &zn, 1, &mp));
+  GPM_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return GPM_OK;
+}
+
+// Plans, filters roots, tunes and lays out a batch.
+int prepare_batch(gpm_graph* g, const gpm_query* queries, int n_queries, const gpm_tuning& tun,
+                  MatchBatch* b) {
+  std::vector<QPlan> plans;
+  for (int qi = 0; qi < n_queries; ++qi) {
+    const gpm_query& gq = queries[qi];
+    HostQuery hq;
+    hq.n = gq.n;
+    hq.adj.assign(gq.n, {});
+    hq.label.assign(gq.labels, gq.labels + gq.n);
+    for (uint32_t e = 0; e < gq.n_edges; ++e) {
+      const uint32_t a = gq.edges[2 * e], c = gq.edges[2 * e + 1];
+      hq.adj[a].push_back(c);
+      hq.adj[c].push_back(a);
+    }
+    for (auto& l : hq.adj) {
+      std::sort(l.begin(), l.end());
+      l.erase(std::unique(l.begin(), l.end()), l.end());
+    }
+    QPlan p;
+    if (!make_plan(g, hq, tun.policy, &p)) continue;  // a label the graph lacks: count 0
+    p.qid = (uint32_t)qi;
+    plans.push_back(p);
+  }
+  b->n_queries = n_queries;
+  b->nq = (uint32_t)plans.size();
+  const uint32_t nq = b->nq;
+  uint64_t root_total = 0;
+  for (auto& p : plans) {
+    p.root_off = (uint32_t)root_total;
+    root_total += g->h_lvoff[p.label[0] + 1] - g->h_lvoff[p.label[0]];
+  }
+  if (root_total >= (1ull << 32)) return fail(GPM_EINVAL, "root candidate total too large");
+  auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t plan_bytes = align(std::max<size_t>(nq, 1) * sizeof(QPlan));
+  const size_t root_bytes = align(std::max<uint64_t>(root_total, 1) * 4);
+  const size_t count_bytes = align((size_t)(n_queries + 1) * 8);
+  const size_t cyc_bytes = align((size_t)(nq + 1) * 8);
+  const size_t small_bytes = align((size_t)(2 * nq + 8) * 4);
+  const size_t lvoff_bytes = align(g->h_lvoff.size() * 4);
+  const size_t total = plan_bytes + root_bytes + count_bytes + cyc_bytes + small_bytes + lvoff_bytes;
+  if (hipMalloc(&b->mem, total) != hipSuccess) return fail(GPM_ENOMEM, "batch buffers (%zu B)", total);
+  char* p = static_cast<char*>(b->mem);
+  b->d_plans = reinterpret_cast<QPlan*>(p);
+  p += plan_bytes;
+  b->d_roots = reinterpret_cast<uint32_t*>(p);
+  p += root_bytes;
+  b->d_counts = reinterpret_cast<unsigned long long*>(p);
+  p += count_bytes;
+  b->d_cycles = reinterpret_cast<unsigned long long*>(p);
+  p += cyc_bytes;
+  b->d_rootcnt = reinterpret_cast<unsigned int*>(p);
+  b->d_stride = b->d_rootcnt + nq + 1;
+  b->d_queue = b->d_stride + nq + 1;
+  p += small_bytes;
+  uint32_t* d_lvoff = reinterpret_cast<uint32_t*>(p);
+  b->dg = DevGraph{g->off, g->ladj, g->loff, g->n_labels};
+  const int cus = device_cus(g);
+  b->grid = (unsigned)cus * 8;
+  if (nq == 0) return GPM_OK;
+  GPM_HIP(hipMemcpy(d_lvoff, g->h_lvoff.data(), g->h_lvoff.size() * 4, hipMemcpyHostToDevice));
+  GPM_HIP(hipMemcpy(b->d_plans, plans.data(), (size_t)nq * sizeof(QPlan), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(zero_kernel, dim3((nq + 64) / 64 + 1), dim3(64), 0, g->stream, b->d_cycles,
+                     (int)nq, b->d_rootcnt, (int)nq);
+  uint32_t maxlab = 0;
+  for (auto& q : plans)
+    maxlab = std::max(maxlab, g->h_lvoff[q.label[0] + 1] - g->h_lvoff[q.label[0]]);
+  dim3 rgrid((unsigned)std::max<uint32_t>(1, std::min<uint32_t>((maxlab + kBlock - 1) / kBlock, 256)),
+             nq);
+  hipLaunchKernelGGL(roots_kernel, rgrid, dim3(kBlock), 0, g->stream, b->dg, g->lverts, d_lvoff,
+                     b->d_plans, b->d_roots, b->d_rootcnt);
+  GPM_HIP(hipGetLastError());
+  std::vector<unsigned int> rootcnt(nq);
+  GPM_HIP(hipMemcpyAsync(rootcnt.data(), b->d_rootcnt, nq * 4, hipMemcpyDeviceToHost, g->stream));
+  GPM_HIP(hipStreamSynchronize(g->stream));
+  for (uint32_t i = 0; i < nq; ++i) plans[i].n_roots = rootcnt[i];
+  // Autotuner: a probe over a sample of every query's roots measures cycles
+  // per root.
+  std::vector<double> cost_per_root(nq, 1.0);
+  std::vector<uint32_t> stride(nq, 1);
+  if (tun.autotune) {
+    const uint32_t kSample = 32;
+    uint32_t t = 0;
+    for (uint32_t i = 0; i < nq; ++i) {
+      stride[i] = std::max<uint32_t>(1, plans[i].n_roots / kSample);
+      plans[i].task_off = t;
+      t += plans[i].n_roots ? (plans[i].n_roots + stride[i] - 1) / stride[i] : 0;
+    }
+    const uint32_t probe_tasks = t;
+    GPM_HIP(hipMemcpy(b->d_plans, plans.data(), (size_t)nq * sizeof(QPlan), hipMemcpyHostToDevice));
+    GPM_HIP(hipMemcpy(b->d_stride, stride.data(), nq * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(zero_kernel, dim3((nq + 64) / 64 + 1), dim3(64), 0, g->stream, b->d_cycles,
+                       (int)nq, b->d_queue, 4);
+    hipLaunchKernelGGL(match_kernel<true>, dim3(b->grid), dim3(kBlock), 0, g->stream, b->dg,
+                       b->d_plans, nq, probe_tasks, b->d_roots, b->d_stride, b->d_queue,
+                       b->d_counts, b->d_cycles);
+    GPM_HIP(hipGetLastError());
+    std::vector<unsigned long long> cyc(nq);
+    GPM_HIP(hipMemcpyAsync(cyc.data(), b->d_cycles, nq * 8, hipMemcpyDeviceToHost, g->stream));
+    GPM_HIP(hipStreamSynchronize(g->stream));
+    for (uint32_t i = 0; i < nq; ++i) {
+      const uint32_t sampled = plans[i].n_roots ? (plans[i].n_roots + stride[i] - 1) / stride[i] : 0;
+      cost_per_root[i] = sampled ? std::max(1.0, (double)cyc[i] / sampled) : 1.0;
+    }
+  }
+  // Chunks (roots per task) and layout, heaviest queries first when tuned.
+  const double kTaskCycles = 2.0e5;
+  std::vector<uint32_t> perm(nq);
+  std::iota(perm.begin(), perm.end(), 0u);
+  if (tun.autotune || tun.policy == GPM_SCHED_LEARNED)
+    std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t c) {
+      return cost_per_root[a] * plans[a].n_roots > cost_per_root[c] * plans[c].n_roots;
+    });
+  std::vector<QPlan> laid;
+  laid.reserve(nq);
+  uint32_t t = 0;
+  for (uint32_t i : perm) {
+    QPlan q = plans[i];
+    if (tun.chunk)
+      q.chunk = tun.chunk;
+    else if (tun.autotune)
+      q.chunk = (uint32_t)std::min(256.0, std::max(1.0, kTaskCycles / cost_per_root[i]));
+    else
+      q.chunk = 8;
+    q.task_off = t;
+    t += q.n_roots ? (q.n_roots + q.chunk - 1) / q.chunk : 0;
+    laid.push_back(q);
+  }
+  b->n_tasks = t;
+  GPM_HIP(hipMemcpy(b->d_plans, laid.data(), (size_t)nq * sizeof(QPlan), hipMemcpyHostToDevice));
+  if (tun.use_graph) return build_graph(b);
+  return GPM_OK;
+}
+
+}  // namespace
+
+namespace gpm {
+void free_match_cache(gpm_graph* g) {
+  free_batch(g->match_cache);
+  g->match_cache = nullptr;
+}
+}  // namespace gpm
+
+extern "C" int gpm_match_batch(gpm_graph* g, const gpm_query* queries, int n_queries,
+                               const gpm_tuning* tuning, uint64_t* per_query) {
+  clear_error();
+  GPM_CHECK_ARG(g && per_query && (queries || n_queries == 0), "null pointer");
+  GPM_CHECK_ARG(n_queries >= 0, "negative query count");
+  GPM_CHECK_ARG(g->flags & GPM_GRAPH_SYMMETRIC, "matching needs a SYMMETRIC graph");
+  GPM_CHECK_ARG(g->labels && g->ladj, "matching needs a labeled graph");
+  gpm_tuning tun{};
+  tun.autotune = 1;
+  tun.use_graph = 1;
+  if (tuning) tun = *tuning;
+  GPM_CHECK_ARG(tun.policy == GPM_SCHED_DEGREE || tun.policy == GPM_SCHED_LEARNED,
+                "unknown policy %d", tun.policy);
+  for (int qi = 0; qi < n_queries; ++qi) {
+    const gpm_query& gq = queries[qi];
+    GPM_CHECK_ARG(gq.n >= 1 && gq.n <= (uint32_t)kMaxQ, "query %d: 1 <= n <= 16 (got %u)", qi,
+                  gq.n);
+    GPM_CHECK_ARG(gq.labels && (gq.n_edges == 0 || gq.edges), "query %d: null arrays", qi);
+    HostQuery hq;
+    hq.n = gq.n;
+    hq.adj.assign(gq.n, {});
+    for (uint32_t e = 0; e < gq.n_edges; ++e) {
+      const uint32_t a = gq.edges[2 * e], c = gq.edges[2 * e + 1];
+      GPM_CHECK_ARG(a < gq.n && c < gq.n && a != c, "query %d: bad edge %u (%u, %u)", qi, e, a, c);
+      hq.adj[a].push_back(c);
+      hq.adj[c].push_back(a);
+    }
+    GPM_CHECK_ARG(connected(hq), "query %d is not connected", qi);
+  }
+  for (int i = 0; i < n_queries; ++i) per_query[i] = 0;
+  if (n_queries == 0) return GPM_OK;
+  const uint64_t key = batch_key(queries, n_queries, tun);
+  MatchBatch* b = static_cast<MatchBatch*>(g->match_cache);
+  if (!b || b->key != key) {
+    free_match_cache(g);
+    b = new MatchBatch();
+    int st = prepare_batch(g, queries, n_queries, tun, b);
+    if (st != GPM_OK) {
+      free_batch(b);
+      return st;
+    }
+    b->key = key;
+    g->match_cache = b;
+  }
+  if (b->nq == 0) return GPM_OK;
+  int st;
+  if ((st = timing_begin(g)) != GPM_OK) return st;
+  if ((st = launch_batch(g, b)) != GPM_OK) return st;
+  timing_mark(g, "match_batch");
+  if ((st = timing_end(g)) != GPM_OK) return st;
+  std::vector<unsigned long long> cnt(n_queries);
+  GPM_HIP(hipMemcpyAsync(cnt.data(), b->d_counts, (size_t)n_queries * 8, hipMemcpyDeviceToHost,
+                         g->stream));
+  GPM_HIP(hipStreamSynchronize(g->stream));
+  for (int i = 0; i < n_queries; ++i) per_query[i] = cnt[i];
+  return GPM_OK;
+}

@@ -6,10 +6,6 @@ using gpm::fail;
 
 extern "C" {
 
-int gpm_match_batch(gpm_graph*, const gpm_query*, int, const gpm_tuning*, uint64_t*) {
-  return fail(GPM_EUNSUPPORTED, "gpm_match_batch: not built yet");
-}
-
 int gpm_vertex_features(gpm_graph*, gpm_graph*, float*) {
   return fail(GPM_EUNSUPPORTED, "gpm_vertex_features: not built yet");
 }

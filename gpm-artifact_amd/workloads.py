@@ -1,0 +1,97 @@
+"""Seeded synthetic inputs of the benchmark configurations.
+
+  tc_graph()            Erdos-Renyi n=16384, M=131072 (triangle counting)
+  clique_graph(scale)   R-MAT scale s, edge factor 16, (.57,.19,.19,.05)
+  labeled_graph(scale)  the same R-MAT with 8 uniform vertex labels
+  named_queries(seed)   triangle, 4-cycle, diamond, three-star and the
+                        6-vertex pyramid (wheel W6: hub + 5-cycle), labels
+                        seeded uniform in [0, 8)
+  sampled_queries(...)  connected queries of 4..10 vertices cut out of the
+                        data graph by seeded random expansion (so every query
+                        has at least one embedding), with the data labels
+"""
+from __future__ import annotations
+
+import numpy as np
+
+import gpm_amd as G
+
+N_LABELS = 8
+SEEDS = {"tc": 20240601, "clique": 20240520, "labeled": 20240607, "queries": 7,
+         "batch": 20240613}
+
+NAMED_SHAPES = {
+    "triangle": (3, [(0, 1), (1, 2), (0, 2)]),
+    "4-cycle": (4, [(0, 1), (1, 2), (2, 3), (3, 0)]),
+    "diamond": (4, [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3)]),
+    "three-star": (4, [(0, 1), (0, 2), (0, 3)]),
+    "pyramid-6": (6, [(0, 1), (0, 2), (0, 3), (0, 4), (0, 5),
+                      (1, 2), (2, 3), (3, 4), (4, 5), (5, 1)]),
+}
+
+
+def tc_graph():
+    return G.er_graph(16384, 131072, SEEDS["tc"])
+
+
+def clique_graph(scale: int = 20):
+    return G.rmat_graph(scale, 16, SEEDS["clique"])
+
+
+def labeled_graph(scale: int, seed: int = SEEDS["labeled"]):
+    g = G.rmat_graph(scale, 16, seed)
+    return g, G.gen_labels(g.n, N_LABELS, seed + 1)
+
+
+def named_queries(seed: int = SEEDS["queries"]):
+    rng = np.random.default_rng(seed)
+    out = []
+    for name, (n, edges) in NAMED_SHAPES.items():
+        labels = rng.integers(0, N_LABELS, n).astype(np.uint32)
+        out.append((name, (n, np.asarray(edges, dtype=np.uint32), labels)))
+    return out
+
+
+def sampled_queries(g: G.HostCSR, labels: np.ndarray, count: int, seed: int,
+                    min_size: int = 4, max_size: int = 10, max_degree: int = 64,
+                    extra_edge_prob: float = 1.0):
+    """Connected queries cut out of g.
+
+    A start vertex of degree in [2, max_degree] is drawn; the query grows by
+    adding a random neighbour (degree <= max_degree) of a random member until
+    it has the drawn size; the query keeps every data edge among its members
+    with probability extra_edge_prob (spanning-tree edges always), and the
+    members' labels.  Deterministic for a given seed.
+    """
+    rng = np.random.default_rng(seed)
+    deg = g.degrees()
+    ok = np.nonzero((deg >= 2) & (deg <= max_degree))[0]
+    out = []
+    while len(out) < count:
+        size = int(rng.integers(min_size, max_size + 1))
+        start = int(ok[rng.integers(0, len(ok))])
+        members = [start]
+        tree = []
+        tries = 0
+        while len(members) < size and tries < 50 * size:
+            tries += 1
+            a = members[int(rng.integers(0, len(members)))]
+            nb = g.adj[g.offsets[a]:g.offsets[a + 1]]
+            b = int(nb[int(rng.integers(0, len(nb)))])
+            if b in members or deg[b] > max_degree:
+                continue
+            members.append(b)
+            tree.append((members.index(a), len(members) - 1))
+        if len(members) < size:
+            continue
+        idx = {v: i for i, v in enumerate(members)}
+        edges = set((min(a, b), max(a, b)) for a, b in tree)
+        for v in members:
+            for w in g.adj[g.offsets[v]:g.offsets[v + 1]]:
+                w = int(w)
+                if w in idx and v < w and (min(idx[v], idx[w]), max(idx[v], idx[w])) not in edges:
+                    if rng.random() < extra_edge_prob:
+                        edges.add((min(idx[v], idx[w]), max(idx[v], idx[w])))
+        e = np.asarray(sorted(edges), dtype=np.uint32).reshape(-1, 2)
+        out.append((size, e, labels[np.asarray(members)].astype(np.uint32)))
+    return out

@@ -572,51 +572,67 @@ u64 ceci_count(const Csr& g, const u32* labels, const QueryG& q) {
   }
   // Iterative DFS (myceci.cpp:540-668) with the candidate intersection of
   // generateValidCandidates (myceci.cpp:670-724).
-  std::vector<u32> emb(qn, 0);
-  std::vector<char> used(g.n, 0);
-  std::vector<std::vector<u32>> valid(qn);
-  std::vector<u32> idx(qn, 0);
-  std::vector<u32> tmp;
+  // The DFS below each root candidate is independent of the others, so the
+  // roots are shared out over OpenMP threads (the reference runs one thread;
+  // the count is the same).  "Visited" is checked against the embedding
+  // itself (at most 16 entries) instead of an n-sized flag array.
+  const std::vector<u32>& roots = cand[start];
   u64 count = 0;
-  valid[0] = cand[start];
-  int depth = 0;
-  idx[0] = 0;
-  auto gen = [&](int d) {
-    u32 u = order[d];
-    valid[d].clear();
-    auto it = te[u].find(emb[parent[u]]);
-    if (it == te[u].end()) return;
-    valid[d] = it->second;
-    for (size_t j = 0; j < bn[u].size() && !valid[d].empty(); ++j) {
-      auto jt = nte[u][j].find(emb[bn[u][j]]);
-      if (jt == nte[u][j].end()) {
-        valid[d].clear();
-        return;
+#pragma omp parallel reduction(+ : count)
+  {
+    std::vector<u32> emb(qn, 0);
+    std::vector<std::vector<u32>> valid(qn);
+    std::vector<u32> idx(qn, 0);
+    std::vector<u32> tmp;
+    auto gen = [&](int d) {
+      u32 u = order[d];
+      valid[d].clear();
+      auto it = te[u].find(emb[parent[u]]);
+      if (it == te[u].end()) return;
+      valid[d] = it->second;
+      for (size_t j = 0; j < bn[u].size() && !valid[d].empty(); ++j) {
+        auto jt = nte[u][j].find(emb[bn[u][j]]);
+        if (jt == nte[u][j].end()) {
+          valid[d].clear();
+          return;
+        }
+        tmp.resize(std::min(valid[d].size(), jt->second.size()));
+        u32 k = isect(valid[d].data(), (u32)valid[d].size(), jt->second.data(),
+                      (u32)jt->second.size(), tmp.data());
+        valid[d].assign(tmp.begin(), tmp.begin() + k);
       }
-      tmp.resize(std::min(valid[d].size(), jt->second.size()));
-      u32 k = isect(valid[d].data(), (u32)valid[d].size(), jt->second.data(),
-                    (u32)jt->second.size(), tmp.data());
-      valid[d].assign(tmp.begin(), tmp.begin() + k);
-    }
-  };
-  while (true) {
-    while (idx[depth] < valid[depth].size()) {
-      u32 u = order[depth];
-      u32 v = valid[depth][idx[depth]++];
-      if (used[v]) continue;
-      if (depth == (int)qn - 1) {
+    };
+    auto used = [&](int depth, u32 v) {
+      for (int j = 0; j < depth; ++j)
+        if (emb[order[j]] == v) return true;
+      return false;
+    };
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t r = 0; r < (int64_t)roots.size(); ++r) {
+      if (qn == 1) {
         ++count;
         continue;
       }
-      emb[u] = v;
-      used[v] = 1;
-      ++depth;
-      idx[depth] = 0;
-      gen(depth);
+      emb[order[0]] = roots[r];
+      int depth = 1;
+      idx[1] = 0;
+      gen(1);
+      while (depth >= 1) {
+        while (idx[depth] < valid[depth].size()) {
+          u32 v = valid[depth][idx[depth]++];
+          if (used(depth, v)) continue;
+          if (depth == (int)qn - 1) {
+            ++count;
+            continue;
+          }
+          emb[order[depth]] = v;
+          ++depth;
+          idx[depth] = 0;
+          gen(depth);
+        }
+        --depth;
+      }
     }
-    --depth;
-    if (depth < 0) break;
-    used[emb[order[depth]]] = 0;
   }
   return count;
 }

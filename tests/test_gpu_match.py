@@ -1,0 +1,89 @@
+"""GPU parity of the labeled matcher: the reference's 200 known answers on
+HPRD, and the CECI oracle on seeded labeled graphs (bit-exact per query)."""
+import numpy as np
+import pytest
+
+import graphs
+from test_ceci import load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hprd(gpm):
+    g, labels, qs, expected, _ = load_golden(gpm)
+    dg = gpm.DeviceGraph(g, oriented=False, labels=labels)
+    yield dg, qs, expected
+    dg.close()
+
+
+@pytest.mark.parametrize("autotune,policy,chunk,use_graph", [
+    (1, 0, 0, 1), (0, 0, 4, 0), (1, 1, 0, 1), (0, 1, 1, 1)])
+def test_hprd_known_answers(gpm, hprd, autotune, policy, chunk, use_graph):
+    dg, qs, expected = hprd
+    got = dg.match_batch(qs, autotune=autotune, policy=policy, chunk=chunk, use_graph=use_graph)
+    assert np.array_equal(got, expected)
+
+
+def _random_queries(rng, n_labels):
+    shapes = [
+        (1, []),
+        (2, [(0, 1)]),
+        (3, [(0, 1), (1, 2), (0, 2)]),
+        (4, [(0, 1), (1, 2), (2, 3), (3, 0)]),
+        (4, [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3)]),
+        (4, [(0, 1), (0, 2), (0, 3)]),
+        (5, [(0, 1), (1, 2), (2, 3), (3, 4)]),
+        (6, [(0, 1), (0, 2), (0, 3), (0, 4), (0, 5), (1, 2), (2, 3), (3, 4), (4, 5), (5, 1)]),
+    ]
+    out = []
+    for n, e in shapes:
+        out.append((n, np.asarray(e, dtype=np.uint32).reshape(-1, 2),
+                    rng.integers(0, n_labels, n).astype(np.uint32)))
+    # Repeated labels stress injectivity.
+    out.append((4, np.asarray([(0, 1), (1, 2), (2, 3)], dtype=np.uint32),
+                np.zeros(4, dtype=np.uint32)))
+    return out
+
+
+@pytest.mark.parametrize("seed,n_labels", [(1, 2), (2, 3), (3, 8)])
+def test_random_vs_oracle(gpm, orc, seed, n_labels):
+    rng = np.random.default_rng(seed)
+    g = gpm.rmat_graph(10, 8, 100 + seed)
+    labels = rng.integers(0, n_labels, g.n).astype(np.uint32)
+    qs = _random_queries(rng, n_labels)
+    want = orc.ceci_batch(g.offsets, g.adj, labels, qs)
+    with gpm.DeviceGraph(g, oriented=False, labels=labels, n_labels=n_labels) as dg:
+        for policy in (0, 1):
+            assert np.array_equal(dg.match_batch(qs, policy=policy), want)
+        assert np.array_equal(dg.match_batch(qs, autotune=0, chunk=3, use_graph=0), want)
+
+
+def test_sampled_queries_vs_oracle(gpm, orc):
+    import workloads
+    g, labels = workloads.labeled_graph(12, seed=5)
+    qs = workloads.sampled_queries(g, labels, 24, seed=9, max_size=8)
+    want = orc.ceci_batch(g.offsets, g.adj, labels, qs)
+    assert np.all(want >= 1)  # cut out of the data graph
+    with gpm.DeviceGraph(g, oriented=False, labels=labels, n_labels=8) as dg:
+        assert np.array_equal(dg.match_batch(qs), want)
+        assert np.array_equal(dg.match_batch(qs, policy=1), want)
+
+
+def test_unknown_label_and_errors(gpm):
+    g = graphs.complete(8)
+    labels = np.zeros(8, dtype=np.uint32)
+    with gpm.DeviceGraph(g, oriented=False, labels=labels, n_labels=1) as dg:
+        # Label 5 does not occur: count 0.  K3 into K8 with one label: 8*7*6.
+        got = dg.match_batch([(2, np.array([[0, 1]], dtype=np.uint32), np.array([0, 5], dtype=np.uint32)),
+                              (3, np.array([[0, 1], [1, 2], [0, 2]], dtype=np.uint32),
+                               np.zeros(3, dtype=np.uint32))])
+        assert got.tolist() == [0, 336]
+        with pytest.raises(gpm.GpmError):  # disconnected
+            dg.match_batch([(3, np.array([[0, 1]], dtype=np.uint32), np.zeros(3, dtype=np.uint32))])
+        with pytest.raises(gpm.GpmError):  # too many vertices
+            dg.match_batch([(17, np.array([[i, i + 1] for i in range(16)], dtype=np.uint32),
+                             np.zeros(17, dtype=np.uint32))])
+    with gpm.DeviceGraph(g, oriented=False) as dg:
+        with pytest.raises(gpm.GpmError):  # unlabeled data graph
+            dg.match_batch([(2, np.array([[0, 1]], dtype=np.uint32), np.zeros(2, dtype=np.uint32))])
