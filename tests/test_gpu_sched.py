@@ -1,0 +1,114 @@
+"""GPU checks of the learned scheduler's pieces: per-vertex features (exact
+integer sums held in fp32) and the BF16-MFMA MLP against two references — a
+numpy model that rounds the same operands to bf16 (tight: pins the matrix-core
+operand layouts, where a transposed or misplaced operand shifts every output)
+and a plain fp32 model (loose: bf16 quantisation error only)."""
+import numpy as np
+import pytest
+
+import graphs
+
+pytestmark = pytest.mark.gpu
+
+
+def to_bf16(x):
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def mlp_ref(feat, p, bf16):
+    r = to_bf16 if bf16 else (lambda a: np.asarray(a, dtype=np.float32))
+    x = r((np.log1p(feat.astype(np.float64)) - p["mean"]) / p["std"]).astype(np.float64)
+    h1 = np.maximum(x @ r(p["w1"]).astype(np.float64).T + p["b1"], 0.0)
+    h1 = r(h1).astype(np.float64)
+    h2 = np.maximum(h1 @ r(p["w2"]).astype(np.float64).T + p["b2"], 0.0)
+    return h2 @ p["w3"].astype(np.float64).reshape(64) + p["b3"][0]
+
+
+def random_params(seed):
+    rng = np.random.default_rng(seed)
+    return dict(
+        mean=rng.normal(0, 1, 4).astype(np.float32),
+        std=rng.uniform(0.5, 2.0, 4).astype(np.float32),
+        w1=rng.normal(0, 0.5, (64, 4)).astype(np.float32),
+        b1=rng.normal(0, 0.1, 64).astype(np.float32),
+        w2=rng.normal(0, 0.125, (64, 64)).astype(np.float32),
+        b2=rng.normal(0, 0.1, 64).astype(np.float32),
+        w3=rng.normal(0, 0.125, (1, 64)).astype(np.float32),
+        b3=rng.normal(0, 0.1, 1).astype(np.float32),
+    )
+
+
+def predict(gpm, feat, p):
+    return gpm.mlp_predict(feat, p["mean"], p["std"], p["w1"], p["b1"], p["w2"], p["b2"],
+                           p["w3"], p["b3"])
+
+
+def host_features(sym, dag, labels):
+    deg = np.diff(sym.offsets.astype(np.int64))
+    dout = np.diff(dag.offsets.astype(np.int64))
+    src = np.repeat(np.arange(dag.n), dout)
+    hop = np.bincount(src, weights=dout[dag.adj.astype(np.int64)], minlength=dag.n)
+    return np.stack([deg, dout, hop, labels], axis=1).astype(np.float32)
+
+
+def test_vertex_features(gpm):
+    g = gpm.rmat_graph(13, 16, 777)
+    d = gpm.orient(g)
+    labels = gpm.gen_labels(g.n, 8, 5)
+    with gpm.DeviceGraph(g, oriented=False, labels=labels, n_labels=8) as sym, \
+            gpm.DeviceGraph(d, oriented=True) as dag:
+        got = gpm.vertex_features(sym, dag)
+        want = host_features(g, d, labels)
+        assert np.array_equal(got, want)
+        # Without a DAG the 2-hop column sums symmetric degrees.
+        got = gpm.vertex_features(sym, None)
+        deg = np.diff(g.offsets.astype(np.int64))
+        src = np.repeat(np.arange(g.n), deg)
+        hop = np.bincount(src, weights=deg[g.adj.astype(np.int64)], minlength=g.n)
+        assert np.array_equal(got[:, 1], np.zeros(g.n, np.float32))
+        assert np.array_equal(got[:, 2], hop.astype(np.float32))
+
+
+@pytest.mark.parametrize("n", [1, 15, 16, 37, 4096 + 5])
+def test_mlp_matches_references(gpm, n):
+    p = random_params(100 + n)
+    rng = np.random.default_rng(n)
+    feat = np.stack([rng.integers(0, 3000, n), rng.integers(0, 700, n),
+                     rng.integers(0, 10 ** 6, n), rng.integers(0, 8, n)], axis=1).astype(np.float32)
+    got = predict(gpm, feat, p).astype(np.float64)
+    ref16 = mlp_ref(feat, p, bf16=True)
+    ref32 = mlp_ref(feat, p, bf16=False)
+    scale = np.abs(ref32).mean() + 1e-3
+    # Same bf16 operands, fp32 accumulation in another order: 1e-4 of scale.
+    assert np.max(np.abs(got - ref16)) <= 1e-4 * scale + 1e-5
+    # bf16 weights/activations vs fp32: relative error of a few 2^-9 steps.
+    assert np.max(np.abs(got - ref32)) <= 3e-2 * scale
+
+
+def test_mlp_errors(gpm):
+    p = random_params(1)
+    bad = dict(p, std=np.array([1, 0, 1, 1], np.float32))
+    with pytest.raises(gpm.GpmError):
+        predict(gpm, np.zeros((4, 4), np.float32), bad)
+    with pytest.raises(gpm.GpmError):
+        predict(gpm, np.full((4, 4), np.nan, np.float32), p)
+    assert predict(gpm, np.zeros((0, 4), np.float32), p).shape == (0,)
+
+
+def test_trained_model_orders_roots(gpm):
+    """The shipped cost model ranks the roots of an unseen graph by logged
+    work: Spearman correlation of predicted cost vs plan entries."""
+    from cost_model import CostModel
+    cm = CostModel.load_default(5)
+    g = gpm.rmat_graph(16, 16, 4242)
+    d = gpm.orient(g)
+    with gpm.DeviceGraph(g, oriented=False) as sym, gpm.DeviceGraph(d, oriented=True) as dag:
+        cost = cm.predict(sym, dag)
+        work = dag.clique_root_work(5).astype(np.float64)
+    keep = work > 0
+    rc = np.argsort(np.argsort(cost[keep]))
+    rw = np.argsort(np.argsort(work[keep]))
+    rho = np.corrcoef(rc, rw)[0, 1]
+    assert rho > 0.8, rho
