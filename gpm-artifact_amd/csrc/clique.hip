@@ -228,14 +228,18 @@ __device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t* s_wsum
 // MAXW > 0 keeps row_i & row_j in registers (W <= MAXW).
 template <int K, int MAXW>
 __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, uint32_t d,
-                                                uint32_t W, uint32_t* s_pre,
-                                                uint32_t* s_wsum) {
+                                                uint32_t W, uint32_t part, uint32_t n_parts,
+                                                uint32_t* s_pre, uint32_t* s_wsum) {
   uint64_t c = 0;
-  for (uint32_t c0 = 0; c0 < d; c0 += kBlock) {
-    const uint32_t i0 = c0 + threadIdx.x;
+  // This task counts rows part, part + n_parts, ...: nr of them.
+  const uint32_t nr = d > part ? (d - part + n_parts - 1) / n_parts : 0u;
+  for (uint32_t c0 = 0; c0 < nr; c0 += kBlock) {
+    const uint32_t t0 = c0 + threadIdx.x;
     uint32_t pc = 0;
-    if (i0 < d)
+    if (t0 < nr) {
+      const uint32_t i0 = part + t0 * n_parts;
       for (uint32_t w = 0; w < W; ++w) pc += (uint32_t)__popcll(rows[(size_t)i0 * W + w]);
+    }
     const uint32_t incl = block_incl_scan(pc, s_wsum);
     s_pre[threadIdx.x] = incl;
     __syncthreads();
@@ -243,7 +247,7 @@ __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, 
     for (uint32_t p = threadIdx.x; p < total; p += kBlock) {
       const uint32_t jj = upper_index(s_pre, kBlock, p);
       uint32_t kth = p - (jj ? s_pre[jj - 1] : 0u);
-      const unsigned long long* ri = rows + (size_t)(c0 + jj) * W;
+      const unsigned long long* ri = rows + (size_t)(part + (c0 + jj) * n_parts) * W;
       uint32_t w = 0;
       while (true) {
         const uint32_t pw = (uint32_t)__popcll(ri[w]);
@@ -290,9 +294,10 @@ __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, 
 
 template <int K, int MODE>
 __device__ __forceinline__ uint64_t count_rows(const unsigned long long* rows, uint32_t d,
-                                               uint32_t W, const uint32_t* lens) {
+                                               uint32_t W, uint32_t part, uint32_t n_parts,
+                                               const uint32_t* lens) {
   uint64_t c = 0;
-  for (uint32_t i = threadIdx.x; i < d; i += kBlock) {
+  for (uint32_t i = part + threadIdx.x * n_parts; i < d; i += kBlock * n_parts) {
     const unsigned long long* ri = rows + (size_t)i * W;
     if (MODE == kCount) {
       for (uint32_t w = 0; w < W; ++w) c += __popcll(ri[w]);  // K == 3 only
@@ -337,8 +342,8 @@ __device__ __forceinline__ uint64_t count_rows(const unsigned long long* rows, u
 template <int K, int MODE, bool LDS_ROWS, int MAXW>
 __global__ __launch_bounds__(kBlock) void clique_block_kernel(
     const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
-    const uint32_t* __restrict__ roots, uint32_t n_roots, uint32_t cap,
-    uint32_t table, unsigned int* __restrict__ queue,
+    const uint32_t* __restrict__ roots, const uint32_t* __restrict__ parts, uint32_t n_roots,
+    uint32_t cap, uint32_t table, unsigned int* __restrict__ queue,
     unsigned long long* __restrict__ grows, uint32_t* __restrict__ gmeta,
     unsigned long long* __restrict__ out, unsigned long long* __restrict__ work) {
   extern __shared__ unsigned long long s_dyn[];
@@ -373,6 +378,8 @@ __global__ __launch_bounds__(kBlock) void clique_block_kernel(
     __syncthreads();
     if (r >= n_roots) break;
     const uint32_t u = uniform(roots[r]);
+    const uint32_t pp = parts ? uniform(parts[r]) : (1u << 16);
+    const uint32_t part = pp & 0xFFFFu, n_parts = pp >> 16;
     const uint32_t sb = uniform(off[u]), d = uniform(off[u + 1]) - sb;
     const uint32_t* S = adj + sb;
     const uint32_t W = (d + 63) / 64;
@@ -444,8 +451,8 @@ __global__ __launch_bounds__(kBlock) void clique_block_kernel(
     __syncthreads();
     if (!LDS_ROWS) __threadfence();
     uint64_t c = (MODE == kCount && K >= 4)
-                     ? count_pairs<K, MAXW>(rows, d, W, s_pre, s_wsum)
-                     : count_rows<K, MODE>(rows, d, W, lens);
+                     ? count_pairs<K, MAXW>(rows, d, W, part, n_parts, s_pre, s_wsum)
+                     : count_rows<K, MODE>(rows, d, W, part, n_parts, lens);
     if (MODE == kWork) {
       c = wave_sum_u64(c);
       if (lane_id() == 0) s_red[threadIdx.x / kWave] = c;
@@ -471,6 +478,13 @@ __global__ void zero_u32_kernel(unsigned int* p, int count) {
   if (i < count) p[i] = 0u;
 }
 
+// LDS hash slots for a tier of this cap (a power of two, >= 2 cap).
+uint32_t table_for(uint32_t cap) {
+  uint32_t table = 1;
+  while (table < 2 * cap) table <<= 1;
+  return table;
+}
+
 size_t block_lds_bytes(uint32_t cap, uint32_t table) {
   const uint32_t W = (cap + 63) / 64;
   return (size_t)cap * W * 8 + (size_t)cap * 8 + (size_t)table * 8;
@@ -488,9 +502,7 @@ uint32_t lds_wide_cap() {
       limit = prop.sharedMemPerBlock;
     uint32_t cap = kBlockCap;
     for (uint32_t c = 1024; c > kBlockCap; c -= 64) {
-      uint32_t table = 1;
-      while (table < 2 * c) table <<= 1;
-      if (block_lds_bytes(c, table) + 4096 <= limit) {
+      if (block_lds_bytes(c, table_for(c)) + 4096 <= limit) {
         cap = c;
         break;
       }
@@ -500,24 +512,29 @@ uint32_t lds_wide_cap() {
   return (uint32_t)cached;
 }
 
+// Blocks of an LDS-rows tier of this cap launched per CU (LDS-limited, <= 8).
+int lds_blocks_per_cu(uint32_t cap) {
+  const size_t lds = block_lds_bytes(cap, table_for(cap));
+  return std::min(8, std::max<int>(1, (int)(160 * 1024 / (lds + 2048))));
+}
+
 // One LDS-rows tier: rows of up to MAXW words, cap = its largest out-degree.
 template <int K, int MODE, int MAXW>
-int launch_lds_tier(gpm_graph* g, const uint32_t* roots, uint32_t n_roots, uint32_t cap,
+int launch_lds_tier(gpm_graph* g, const uint32_t* roots, const uint32_t* parts,
+                    uint32_t n_roots, uint32_t cap,
                     unsigned int* queue, unsigned long long* out, unsigned long long* work,
                     const char* name) {
   const int cus = device_cus(g);
-  uint32_t table = 1;
-  while (table < 2 * cap) table <<= 1;
+  const uint32_t table = table_for(cap);
   const size_t lds = block_lds_bytes(cap, table);
   auto kern = clique_block_kernel<K, MODE, true, MAXW>;
   if (lds > 48 * 1024)
     GPM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int per_cu = std::max<int>(1, (int)(160 * 1024 / (lds + 2048)));
   const unsigned blocks =
-      (unsigned)std::min<uint64_t>(n_roots, (uint64_t)cus * std::min(per_cu, 8));
+      (unsigned)std::min<uint64_t>(n_roots, (uint64_t)cus * lds_blocks_per_cu(cap));
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), lds, g->stream, g->off, g->adj, roots,
-                     n_roots, cap, table, queue, nullptr, nullptr, out, work);
+                     parts, n_roots, cap, table, queue, nullptr, nullptr, out, work);
   GPM_HIP(hipGetLastError());
   if (MODE == kCount) timing_mark(g, name);
   return GPM_OK;
@@ -541,19 +558,21 @@ int launch_tiers(gpm_graph* g, const RootTiers* t, unsigned long long* out,
     unsigned long long* grows = static_cast<unsigned long long*>(g->scratch);
     uint32_t* gmeta = reinterpret_cast<uint32_t*>(static_cast<char*>(g->scratch) + row_bytes);
     hipLaunchKernelGGL((clique_block_kernel<K, MODE, false, 0>), dim3(blocks), dim3(kBlock), 0,
-                       g->stream, g->off, g->adj, t->d_big, t->n_big, cap, 0u, q + 2, grows,
+                       g->stream, g->off, g->adj, t->d_big, t->d_big_parts, t->n_big, cap, 0u,
+                       q + 2, grows,
                        gmeta, out, work);
     GPM_HIP(hipGetLastError());
     if (MODE == kCount) timing_mark(g, "clique_hbm_block");
   }
   if (t->n_block) {
-    int st = launch_lds_tier<K, MODE, 8>(g, t->d_block, t->n_block, t->max_block_deg, q + 1,
-                                         out, work, "clique_lds_block");
+    int st = launch_lds_tier<K, MODE, 8>(g, t->d_block, t->d_block_parts, t->n_block,
+                                         t->max_block_deg, q + 1, out, work, "clique_lds_block");
     if (st != GPM_OK) return st;
   }
   if (t->n_block2) {
-    int st = launch_lds_tier<K, MODE, 16>(g, t->d_block2, t->n_block2, t->max_block2_deg,
-                                          q + 3, out, work, "clique_lds_block_wide");
+    int st = launch_lds_tier<K, MODE, 16>(g, t->d_block2, t->d_block2_parts, t->n_block2,
+                                          t->max_block2_deg, q + 3, out, work,
+                                          "clique_lds_block_wide");
     if (st != GPM_OK) return st;
   }
   if (t->n_wave) {
@@ -618,7 +637,13 @@ int gpm_clique(gpm_graph* g, int k, const gpm_sched* sched, uint64_t* out) {
   GPM_CHECK_ARG(!sched || sched->policy != GPM_SCHED_LEARNED || sched->root_cost,
                 "LEARNED policy needs root_cost");
   TierRequest req = request_from(sched, (uint32_t)(k - 1));
-  if (req.policy == GPM_SCHED_LEARNED) req.heavy_cost = learned_heavy_cost(req.cost, g->n);
+  if (req.policy == GPM_SCHED_LEARNED) {
+    req.heavy_cost = learned_heavy_cost(req.cost, g->n);
+    const uint32_t cus = (uint32_t)device_cus(g);
+    req.resident_block = cus * lds_blocks_per_cu(req.block_cap);
+    req.resident_block2 = cus * lds_blocks_per_cu(req.block2_cap);
+    req.resident_big = cus;
+  }
   const RootTiers* t = nullptr;
   int st = get_tiers(g, req, &t);
   if (st != GPM_OK) return st;

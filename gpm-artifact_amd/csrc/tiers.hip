@@ -1,5 +1,6 @@
 // Root tier construction and caching (see tiers.hpp).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <numeric>
 
@@ -22,8 +23,9 @@ std::string key_of(const TierRequest& r, uint32_t n) {
   if (r.n_root_list)
     cs ^= checksum(reinterpret_cast<const float*>(r.root_list), r.n_root_list) * 31 +
           r.n_root_list;
-  std::snprintf(buf, sizeof(buf), "%u/%u/%u/%u/%u/%u/%d/%u/%g/%llx", r.min_deg, r.wave_cap,
-                r.block_cap, r.block2_cap, r.root_begin, r.root_end, r.policy, r.chunk, r.heavy_cost,
+  std::snprintf(buf, sizeof(buf), "%u/%u/%u/%u/%u/%u/%d/%u/%g/%u/%u/%u/%llx", r.min_deg,
+                r.wave_cap, r.block_cap, r.block2_cap, r.root_begin, r.root_end, r.policy, r.chunk,
+                r.heavy_cost, r.resident_block, r.resident_block2, r.resident_big,
                 (unsigned long long)cs);
   return buf;
 }
@@ -34,6 +36,13 @@ int upload(const std::vector<uint32_t>& v, uint32_t** d) {
   if (hipMemcpy(*d, v.data(), v.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
     return fail(GPM_EHIP, "tier list upload");
   return GPM_OK;
+}
+
+void release(RootTiers& t) {
+  for (uint32_t* p : {t.d_wave, t.d_block, t.d_block2, t.d_big, t.d_block_parts, t.d_block2_parts,
+                      t.d_big_parts})
+    (void)hipFree(p);
+  (void)hipFree(t.queues);
 }
 
 }  // namespace
@@ -81,6 +90,8 @@ int get_tiers(gpm_graph* g, const TierRequest& req, const RootTiers** out) {
       maxg = std::max(maxg, d);
     }
   }
+  std::vector<uint32_t> block_parts, block2_parts, big_parts;
+  uint32_t n_split = 0;
   if (learned) {
     // Heaviest predicted roots first (longest-processing-time order), so the
     // expensive subtrees start early and the tail of the launch is short.
@@ -88,9 +99,48 @@ int get_tiers(gpm_graph* g, const TierRequest& req, const RootTiers** out) {
       return req.cost[a] > req.cost[b] || (req.cost[a] == req.cost[b] && a < b);
     };
     std::stable_sort(wave.begin(), wave.end(), by_cost);
-    std::stable_sort(block.begin(), block.end(), by_cost);
-    std::stable_sort(block2.begin(), block2.end(), by_cost);
-    std::stable_sort(big.begin(), big.end(), by_cost);
+    // Block tiers: a root predicted to cost more than a quarter of one
+    // resident block's fair share of its tier is dealt out as up to
+    // kMaxParts tasks (each part counts every n-th row of the root's local
+    // DAG; the rows themselves are rebuilt per part).  Tasks are then ordered
+    // by predicted cost per part.
+    auto split = [&](std::vector<uint32_t>& tier, std::vector<uint32_t>& parts,
+                     uint32_t resident) {
+      double total = 0.0;
+      for (const uint32_t v : tier) total += std::max(0.f, req.cost[v]);
+      const double target = resident ? total / (4.0 * resident) : 0.0;
+      struct Task {
+        float cost;
+        uint32_t v, part;
+      };
+      std::vector<Task> tasks;
+      tasks.reserve(tier.size());
+      bool any = false;
+      for (const uint32_t v : tier) {
+        const uint32_t d = g->h_off[v + 1] - g->h_off[v];
+        uint32_t np = 1;
+        const double c = std::max(0.f, req.cost[v]);
+        if (target > 0.0 && c > target)
+          np = (uint32_t)std::min<double>(kMaxParts, std::ceil(c / target));
+        np = std::max(1u, std::min(np, d / kMinRowsPerPart));
+        if (np > 1) {
+          any = true;
+          ++n_split;
+        }
+        for (uint32_t p = 0; p < np; ++p) tasks.push_back({(float)(c / np), v, p | (np << 16)});
+      }
+      std::stable_sort(tasks.begin(), tasks.end(), [](const Task& a, const Task& b) {
+        return a.cost > b.cost || (a.cost == b.cost && (a.v < b.v || (a.v == b.v && a.part < b.part)));
+      });
+      tier.clear();
+      for (const Task& t : tasks) {
+        tier.push_back(t.v);
+        if (any) parts.push_back(t.part);
+      }
+    };
+    split(block, block_parts, req.resident_block);
+    split(block2, block2_parts, req.resident_block2);
+    split(big, big_parts, req.resident_big);
   }
   RootTiers t;
   t.n_wave = (uint32_t)wave.size();
@@ -106,18 +156,17 @@ int get_tiers(gpm_graph* g, const TierRequest& req, const RootTiers** out) {
   if ((st = upload(block, &t.d_block)) != GPM_OK) return st;
   if ((st = upload(block2, &t.d_block2)) != GPM_OK) return st;
   if ((st = upload(big, &t.d_big)) != GPM_OK) return st;
+  if ((st = upload(block_parts, &t.d_block_parts)) != GPM_OK) return st;
+  if ((st = upload(block2_parts, &t.d_block2_parts)) != GPM_OK) return st;
+  if ((st = upload(big_parts, &t.d_big_parts)) != GPM_OK) return st;
+  t.n_split = n_split;
   if (hipMalloc(&t.queues, 4 * sizeof(unsigned int)) != hipSuccess)
     return fail(GPM_ENOMEM, "work queues");
   g->tier_cache.emplace_back(key, t);
   *out = &g->tier_cache.back().second;
   // Keep the cache small: learned requests with changing costs would grow it.
   if (g->tier_cache.size() > 16) {
-    RootTiers& old = g->tier_cache.front().second;
-    hipFree(old.d_wave);
-    (void)hipFree(old.d_block);
-    (void)hipFree(old.d_block2);
-    (void)hipFree(old.d_big);
-    hipFree(old.queues);
+    release(g->tier_cache.front().second);
     g->tier_cache.erase(g->tier_cache.begin());
     *out = &g->tier_cache.back().second;
   }
@@ -133,13 +182,7 @@ int get_tc_tiers(gpm_graph* g, uint32_t wave_cap, const RootTiers** out) {
 }
 
 void free_tiers(gpm_graph* g) {
-  for (auto& kv : g->tier_cache) {
-    hipFree(kv.second.d_wave);
-    (void)hipFree(kv.second.d_block);
-    (void)hipFree(kv.second.d_block2);
-    (void)hipFree(kv.second.d_big);
-    hipFree(kv.second.queues);
-  }
+  for (auto& kv : g->tier_cache) release(kv.second);
   g->tier_cache.clear();
 }
 

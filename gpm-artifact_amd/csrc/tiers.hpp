@@ -25,6 +25,13 @@ struct RootTiers {
   uint32_t* d_big = nullptr;    // roots served by one block, rows in HBM
   uint32_t n_big = 0;
   uint32_t max_big_deg = 0;
+  // Learned policy: a heavy root may appear several times in a block tier,
+  // once per part; parts[r] = part | (n_parts << 16) says which share of the
+  // root's rows task r counts.  nullptr = every task is a whole root.
+  uint32_t* d_block_parts = nullptr;
+  uint32_t* d_block2_parts = nullptr;
+  uint32_t* d_big_parts = nullptr;
+  uint32_t n_split = 0;            // roots split into more than one part
   unsigned int* queues = nullptr;  // 4 work-queue counters
   uint32_t wave_chunk = 8;         // roots per wave fetch
 };
@@ -41,7 +48,15 @@ struct TierRequest {
   const float* cost = nullptr;  // per-vertex predicted cost (learned)
   uint32_t chunk = 0;         // 0 = automatic
   float heavy_cost = 0.f;     // learned: wave roots above this go to blocks
+  // Learned: blocks of each block tier resident at once on the device; a
+  // root predicted to cost more than a quarter of a block's fair share of
+  // its tier is split into parts (0 = never split).
+  uint32_t resident_block = 0, resident_block2 = 0, resident_big = 0;
 };
+
+// Most parts a root is split into, and fewest rows a part is given.
+constexpr uint32_t kMaxParts = 64;
+constexpr uint32_t kMinRowsPerPart = 8;
 
 // Returns cached tiers for (graph, request); builds and uploads on a miss.
 int get_tiers(gpm_graph* g, const TierRequest& req, const RootTiers** out);

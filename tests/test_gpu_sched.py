@@ -112,3 +112,34 @@ def test_trained_model_orders_roots(gpm):
     rw = np.argsort(np.argsort(work[keep]))
     rho = np.corrcoef(rc, rw)[0, 1]
     assert rho > 0.8, rho
+
+
+@pytest.mark.parametrize("k", [3, 4, 5])
+def test_learned_split_counts_identical(gpm, k):
+    """LEARNED orders roots by cost and deals heavy ones out in parts (every
+    n-th row of the root's local DAG per part); counts must not change."""
+    g = gpm.rmat_graph(15, 16, 99)
+    d = gpm.orient(g)
+    deg = np.diff(d.offsets.astype(np.int64)).astype(np.float32)
+    rng = np.random.default_rng(k)
+    with gpm.DeviceGraph(d, oriented=True) as dag:
+        plain = dag.clique(k)
+        for cost in (deg ** 3, deg, rng.random(d.n).astype(np.float32) * 1e6,
+                     np.zeros(d.n, np.float32)):
+            assert dag.clique(k, policy=gpm.SCHED_LEARNED, cost=cost) == plain
+
+
+@pytest.mark.parametrize("n", [40, 300, 700, 1100])
+def test_learned_split_complete(gpm, n):
+    # One root of K_n carries the whole count: it is split into the most
+    # parts, in every block tier (LDS, wide LDS, HBM rows).
+    from math import comb
+    with gpm.DeviceGraph(gpm.orient(graphs.complete(n)), oriented=True) as dag:
+        cost = np.zeros(n, np.float32)
+        cost[np.argmax(np.diff(dag_offsets(gpm, n)))] = 1e9
+        for k in (3, 4) + ((5,) if n <= 300 else ()):
+            assert dag.clique(k, policy=gpm.SCHED_LEARNED, cost=cost) == comb(n, k)
+
+
+def dag_offsets(gpm, n):
+    return gpm.orient(graphs.complete(n)).offsets.astype(np.int64)
