@@ -95,6 +95,13 @@ int device_cus(const gpm_graph* g) {
   return cached_cus;
 }
 
+int h2d(gpm_graph* g, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return GPM_OK;
+  GPM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g->stream));
+  GPM_HIP(hipStreamSynchronize(g->stream));
+  return GPM_OK;
+}
+
 int ensure_scratch(gpm_graph* g, size_t bytes) {
   if (g->scratch_bytes >= bytes) return GPM_OK;
   if (g->scratch) {

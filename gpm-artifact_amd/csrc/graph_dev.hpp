@@ -59,6 +59,12 @@ struct gpm_graph {
 
 namespace gpm {
 
+// Host-to-device copy ordered on the handle's stream and complete on return.
+// (A plain hipMemcpy runs on the null stream, which does not order against
+// the handle's non-blocking stream, and may return before a pageable
+// source's DMA lands: a kernel launched next on the handle's stream could
+// read the old contents.)
+int h2d(gpm_graph* g, void* dst, const void* src, size_t bytes);
 // Grows g->scratch to at least `bytes`; returns status.
 int ensure_scratch(gpm_graph* g, size_t bytes);
 // Builds g->src (COO sources) if missing.

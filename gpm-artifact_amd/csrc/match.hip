@@ -521,8 +521,9 @@ int prepare_batch(gpm_graph* g, const gpm_query* queries, int n_queries, const g
   const int cus = device_cus(g);
   b->grid = (unsigned)cus * 8;
   if (nq == 0) return GPM_OK;
-  GPM_HIP(hipMemcpy(d_lvoff, g->h_lvoff.data(), g->h_lvoff.size() * 4, hipMemcpyHostToDevice));
-  GPM_HIP(hipMemcpy(b->d_plans, plans.data(), (size_t)nq * sizeof(QPlan), hipMemcpyHostToDevice));
+  int st;
+  if ((st = h2d(g, d_lvoff, g->h_lvoff.data(), g->h_lvoff.size() * 4)) != GPM_OK) return st;
+  if ((st = h2d(g, b->d_plans, plans.data(), (size_t)nq * sizeof(QPlan))) != GPM_OK) return st;
   hipLaunchKernelGGL(zero_kernel, dim3((nq + 64) / 64 + 1), dim3(64), 0, g->stream, b->d_cycles,
                      (int)nq, b->d_rootcnt, (int)nq);
   uint32_t maxlab = 0;
@@ -550,8 +551,8 @@ int prepare_batch(gpm_graph* g, const gpm_query* queries, int n_queries, const g
       t += plans[i].n_roots ? (plans[i].n_roots + stride[i] - 1) / stride[i] : 0;
     }
     const uint32_t probe_tasks = t;
-    GPM_HIP(hipMemcpy(b->d_plans, plans.data(), (size_t)nq * sizeof(QPlan), hipMemcpyHostToDevice));
-    GPM_HIP(hipMemcpy(b->d_stride, stride.data(), nq * 4, hipMemcpyHostToDevice));
+    if ((st = h2d(g, b->d_plans, plans.data(), (size_t)nq * sizeof(QPlan))) != GPM_OK) return st;
+    if ((st = h2d(g, b->d_stride, stride.data(), (size_t)nq * 4)) != GPM_OK) return st;
     hipLaunchKernelGGL(zero_kernel, dim3((nq + 64) / 64 + 1), dim3(64), 0, g->stream, b->d_cycles,
                        (int)nq, b->d_queue, 4);
     hipLaunchKernelGGL(match_kernel<true>, dim3(b->grid), dim3(kBlock), 0, g->stream, b->dg,
@@ -590,7 +591,7 @@ int prepare_batch(gpm_graph* g, const gpm_query* queries, int n_queries, const g
     laid.push_back(q);
   }
   b->n_tasks = t;
-  GPM_HIP(hipMemcpy(b->d_plans, laid.data(), (size_t)nq * sizeof(QPlan), hipMemcpyHostToDevice));
+  if ((st = h2d(g, b->d_plans, laid.data(), (size_t)nq * sizeof(QPlan))) != GPM_OK) return st;
   if (tun.use_graph) return build_graph(b);
   return GPM_OK;
 }

@@ -197,7 +197,8 @@ int ensure_rank(gpm_graph* g) {
   for (uint32_t i = 0; i < g->n; ++i) rank[order[i]] = i;
   if (hipMalloc(&g->rank, (size_t)std::max(g->n, 1u) * 4) != hipSuccess)
     return fail(GPM_ENOMEM, "rank array");
-  GPM_HIP(hipMemcpy(g->rank, rank.data(), (size_t)g->n * 4, hipMemcpyHostToDevice));
+  int st = h2d(g, g->rank, rank.data(), (size_t)g->n * 4);
+  if (st != GPM_OK) return st;
   return GPM_OK;
 }
 

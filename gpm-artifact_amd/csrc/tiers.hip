@@ -30,12 +30,10 @@ std::string key_of(const TierRequest& r, uint32_t n) {
   return buf;
 }
 
-int upload(const std::vector<uint32_t>& v, uint32_t** d) {
+int upload(gpm_graph* g, const std::vector<uint32_t>& v, uint32_t** d) {
   if (v.empty()) return GPM_OK;
   if (hipMalloc(d, v.size() * 4) != hipSuccess) return fail(GPM_ENOMEM, "tier list");
-  if (hipMemcpy(*d, v.data(), v.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
-    return fail(GPM_EHIP, "tier list upload");
-  return GPM_OK;
+  return h2d(g, *d, v.data(), v.size() * 4);
 }
 
 void release(RootTiers& t) {
@@ -152,13 +150,13 @@ int get_tiers(gpm_graph* g, const TierRequest& req, const RootTiers** out) {
   t.max_big_deg = maxg;
   t.wave_chunk = req.chunk ? req.chunk : 8;
   int st;
-  if ((st = upload(wave, &t.d_wave)) != GPM_OK) return st;
-  if ((st = upload(block, &t.d_block)) != GPM_OK) return st;
-  if ((st = upload(block2, &t.d_block2)) != GPM_OK) return st;
-  if ((st = upload(big, &t.d_big)) != GPM_OK) return st;
-  if ((st = upload(block_parts, &t.d_block_parts)) != GPM_OK) return st;
-  if ((st = upload(block2_parts, &t.d_block2_parts)) != GPM_OK) return st;
-  if ((st = upload(big_parts, &t.d_big_parts)) != GPM_OK) return st;
+  if ((st = upload(g, wave, &t.d_wave)) != GPM_OK) return st;
+  if ((st = upload(g, block, &t.d_block)) != GPM_OK) return st;
+  if ((st = upload(g, block2, &t.d_block2)) != GPM_OK) return st;
+  if ((st = upload(g, big, &t.d_big)) != GPM_OK) return st;
+  if ((st = upload(g, block_parts, &t.d_block_parts)) != GPM_OK) return st;
+  if ((st = upload(g, block2_parts, &t.d_block2_parts)) != GPM_OK) return st;
+  if ((st = upload(g, big_parts, &t.d_big_parts)) != GPM_OK) return st;
   t.n_split = n_split;
   if (hipMalloc(&t.queues, 4 * sizeof(unsigned int)) != hipSuccess)
     return fail(GPM_ENOMEM, "work queues");

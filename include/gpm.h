@@ -258,8 +258,10 @@ int gpm_match_batch(gpm_graph* g, const gpm_query* queries, int n_queries,
 
 /*
  * Per-vertex features for the work estimator, computed on the device:
- * feat[4*v + {0,1,2,3}] = {deg(v), out-degree in dag, 2-hop size bound
- * sum_{u in N(v)} deg(u), label}.  dag may be NULL (out-degree 0).
+ * feat[4*v + {0,1,2,3}] = {deg(v), out-degree d+(v) in dag, 2-hop size
+ * bound, label (0 without labels)}.  The 2-hop bound is sum_{u in N+(v)} d+(u)
+ * when dag is given (the entries a k-clique root streams) and
+ * sum_{u in N(v)} deg(u) when dag is NULL (then d+(v) reads 0).
  */
 int gpm_vertex_features(gpm_graph* sym, gpm_graph* dag, float* feat);
 
