@@ -206,6 +206,14 @@ int build_label_index(gpm_graph* g, const uint32_t* offsets, const uint32_t* adj
                          g->stream));
   // Vertices grouped by label (counting sort, ids ascending inside a label).
   const uint32_t L = g->n_labels;
+  if (L <= 1024) {
+    g->h_lpair.assign((size_t)L * L, 0.0);
+    for (uint32_t v = 0; v < g->n; ++v) {
+      const uint32_t* lo = loff.data() + (size_t)v * (L + 1);
+      double* row = g->h_lpair.data() + (size_t)labels[v] * L;
+      for (uint32_t l = 0; l < L; ++l) row[l] += lo[l + 1] - lo[l];
+    }
+  }
   g->h_lvoff.assign(L + 1, 0);
   for (uint32_t v = 0; v < g->n; ++v) g->h_lvoff[labels[v] + 1]++;
   for (uint32_t l = 0; l < L; ++l) g->h_lvoff[l + 1] += g->h_lvoff[l];

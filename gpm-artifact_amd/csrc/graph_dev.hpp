@@ -30,6 +30,9 @@ struct gpm_graph {
   // Vertices grouped by label: lverts[h_lvoff[l] .. h_lvoff[l+1]) have label l.
   uint32_t* lverts = nullptr;
   std::vector<uint32_t> h_lvoff;
+  // Adjacency entries per (label of source, label of target), L x L; empty
+  // when L > 1024.  Feeds the matcher's cost estimates.
+  std::vector<double> h_lpair;
   std::vector<uint32_t> h_off;
   std::vector<uint32_t> h_labels;
   hipStream_t stream = nullptr;

@@ -25,9 +25,11 @@
 //    other backward neighbours' label slices by binary search) and descends
 //    into the set bits one by one; the last depth only counts ballots.  All
 //    DFS state is wave-uniform and lives in LDS.
-//  * Autotuner: a probe launch over a sample of every query's roots measures
-//    the cycles one root costs; each query's chunk (roots per task) is set so
-//    a task costs about the same, and queries are laid out heaviest first.
+//  * Autotuner (host): the work one root of a query costs is estimated from
+//    the data graph's label statistics (expected label-slice lengths and
+//    pass rates per depth of the plan); each query's chunk (roots per task)
+//    is set so a task costs about the same, and queries are laid out
+//    heaviest first.
 //  * The tuned launch (counter reset + enumeration) is captured once into a
 //    hipGraph and replayed.
 #include <algorithm>
@@ -212,14 +214,12 @@ __device__ __forceinline__ uint32_t find_plan(const QPlan* plans, uint32_t nq, u
   return lo;
 }
 
-// Persistent enumeration.  PROBE: tasks are sampled roots (one each); the
-// cycles spent are added to cycles[plan] instead of counting.
-template <bool PROBE>
+// Persistent enumeration: every wave pulls tasks (a chunk of one query's
+// root candidates) from one queue until the queue is drained.
 __global__ __launch_bounds__(kBlock) void match_kernel(
     DevGraph g, const QPlan* __restrict__ plans, uint32_t nq, uint32_t n_tasks,
-    const uint32_t* __restrict__ roots, const uint32_t* __restrict__ sample_stride,
-    unsigned int* __restrict__ queue, unsigned long long* __restrict__ counts,
-    unsigned long long* __restrict__ cycles) {
+    const uint32_t* __restrict__ roots, unsigned int* __restrict__ queue,
+    unsigned long long* __restrict__ counts) {
   __shared__ WaveState s_state[kWavesPerBlock];
   WaveState& s = s_state[threadIdx.x / kWave];
   const uint32_t lane = lane_id();
@@ -230,20 +230,11 @@ __global__ __launch_bounds__(kBlock) void match_kernel(
     if (t >= n_tasks) break;
     const uint32_t pi = uniform(find_plan(plans, nq, t));
     const QPlan& p = plans[pi];
-    const uint32_t k = t - p.task_off;
+    const uint32_t r0 = (t - p.task_off) * p.chunk;
+    const uint32_t r1 = min(r0 + p.chunk, p.n_roots);
     uint64_t c = 0;
-    if (PROBE) {
-      const uint32_t r = k * sample_stride[pi];
-      const long long t0 = clock64();
-      c = count_root(g, p, s, uniform(roots[p.root_off + r]));
-      const long long t1 = clock64();
-      if (lane == 0) atomicAdd(&cycles[pi], (unsigned long long)(t1 - t0) + (c & 0));
-    } else {
-      const uint32_t r0 = k * p.chunk;
-      const uint32_t r1 = min(r0 + p.chunk, p.n_roots);
-      for (uint32_t r = r0; r < r1; ++r) c += count_root(g, p, s, uniform(roots[p.root_off + r]));
-      if (lane == 0 && c) atomicAdd(&counts[p.qid], (unsigned long long)c);
-    }
+    for (uint32_t r = r0; r < r1; ++r) c += count_root(g, p, s, uniform(roots[p.root_off + r]));
+    if (lane == 0 && c) atomicAdd(&counts[p.qid], (unsigned long long)c);
   }
 }
 
@@ -379,9 +370,7 @@ struct MatchBatch {
   QPlan* d_plans = nullptr;
   uint32_t* d_roots = nullptr;
   unsigned long long* d_counts = nullptr;
-  unsigned long long* d_cycles = nullptr;
   unsigned int* d_rootcnt = nullptr;
-  unsigned int* d_stride = nullptr;
   unsigned int* d_queue = nullptr;
   uint32_t nq = 0, n_tasks = 0;
   int n_queries = 0;
@@ -425,9 +414,8 @@ int launch_batch(gpm_graph* g, MatchBatch* b) {
   }
   hipLaunchKernelGGL(zero_kernel, dim3((b->n_queries + 64) / 64 + 1), dim3(64), 0, g->stream,
                      b->d_counts, b->n_queries, b->d_queue, 4);
-  hipLaunchKernelGGL(match_kernel<false>, dim3(b->grid), dim3(kBlock), 0, g->stream, b->dg,
-                     b->d_plans, b->nq, b->n_tasks, b->d_roots, b->d_stride, b->d_queue,
-                     b->d_counts, b->d_cycles);
+  hipLaunchKernelGGL(match_kernel, dim3(b->grid), dim3(kBlock), 0, g->stream, b->dg,
+                     b->d_plans, b->nq, b->n_tasks, b->d_roots, b->d_queue, b->d_counts);
   GPM_HIP(hipGetLastError());
   return GPM_OK;
 }
@@ -447,10 +435,10 @@ int build_graph(MatchBatch* b) {
   zp.extra = nullptr;
   hipGraphNode_t zn = nullptr, mn = nullptr;
   GPM_HIP(hipGraphAddKernelNode(&zn, b->graph, nullptr, 0, &zp));
-  void* margs[] = {&b->dg,      &b->d_plans, &b->nq,       &b->n_tasks, &b->d_roots,
-                   &b->d_stride, &b->d_queue, &b->d_counts, &b->d_cycles};
+  void* margs[] = {&b->dg,      &b->d_plans, &b->nq,      &b->n_tasks,
+                   &b->d_roots, &b->d_queue, &b->d_counts};
   hipKernelNodeParams mp{};
-  mp.func = reinterpret_cast<void*>(&match_kernel<false>);
+  mp.func = reinterpret_cast<void*>(&match_kernel);
   mp.gridDim = dim3(b->grid);
   mp.blockDim = dim3(kBlock);
   mp.sharedMemBytes = 0;
@@ -459,6 +447,45 @@ int build_graph(MatchBatch* b) {
   GPM_HIP(hipGraphAddKernelNode(&mn, b->graph, &zn, 1, &mp));
   GPM_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   return GPM_OK;
+}
+
+// Expected work (label-slice entries streamed, times the membership probes
+// each entry takes) of the DFS below one root of plan p, from the data
+// graph's label statistics: at depth d the streamed slice is the anchor's
+// label-d slice (expected length E(l_b, l_d) / |V(l_b)| for a backward
+// neighbour b, the shortest taken), and an entry survives each further
+// backward neighbour b' with probability E(l_b', l_d) / (|V(l_b')| |V(l_d)|).
+double estimate_root_cost(const gpm_graph* g, const QPlan& p) {
+  const uint32_t L = g->n_labels;
+  auto nv = [&](uint32_t l) {
+    return std::max(1.0, (double)(g->h_lvoff[l + 1] - g->h_lvoff[l]));
+  };
+  auto epair = [&](uint32_t a, uint32_t b) {
+    if (!g->h_lpair.empty()) return g->h_lpair[(size_t)a * L + b];
+    return (double)g->m * nv(a) * nv(b) / ((double)g->n * (double)g->n);
+  };
+  double nodes = 1.0, work = 0.0;
+  for (uint32_t d = 1; d < p.n; ++d) {
+    double slice = 1e300;
+    uint32_t anchor = 0;
+    for (uint32_t bn = p.bn[d]; bn; bn &= bn - 1) {
+      const uint32_t b = (uint32_t)__builtin_ctz(bn);
+      const double len = epair(p.label[b], p.label[d]) / nv(p.label[b]);
+      if (len < slice) {
+        slice = len;
+        anchor = b;
+      }
+    }
+    if (slice >= 1e300) slice = 0.0;
+    double pass = 1.0;
+    for (uint32_t bn = p.bn[d] & ~(1u << anchor); bn; bn &= bn - 1) {
+      const uint32_t b = (uint32_t)__builtin_ctz(bn);
+      pass *= std::min(1.0, epair(p.label[b], p.label[d]) / (nv(p.label[b]) * nv(p.label[d])));
+    }
+    work += nodes * slice * (double)__builtin_popcount(p.bn[d]);
+    nodes *= slice * pass;
+  }
+  return std::max(1.0, work);
 }
 
 // Plans, filters roots, tunes and lays out a batch.
@@ -498,10 +525,9 @@ int prepare_batch(gpm_graph* g, const gpm_query* queries, int n_queries, const g
   const size_t plan_bytes = align(std::max<size_t>(nq, 1) * sizeof(QPlan));
   const size_t root_bytes = align(std::max<uint64_t>(root_total, 1) * 4);
   const size_t count_bytes = align((size_t)(n_queries + 1) * 8);
-  const size_t cyc_bytes = align((size_t)(nq + 1) * 8);
-  const size_t small_bytes = align((size_t)(2 * nq + 8) * 4);
+  const size_t small_bytes = align((size_t)(nq + 1 + 4) * 4);
   const size_t lvoff_bytes = align(g->h_lvoff.size() * 4);
-  const size_t total = plan_bytes + root_bytes + count_bytes + cyc_bytes + small_bytes + lvoff_bytes;
+  const size_t total = plan_bytes + root_bytes + count_bytes + small_bytes + lvoff_bytes;
   if (hipMalloc(&b->mem, total) != hipSuccess) return fail(GPM_ENOMEM, "batch buffers (%zu B)", total);
   char* p = static_cast<char*>(b->mem);
   b->d_plans = reinterpret_cast<QPlan*>(p);
@@ -510,11 +536,8 @@ int prepare_batch(gpm_graph* g, const gpm_query* queries, int n_queries, const g
   p += root_bytes;
   b->d_counts = reinterpret_cast<unsigned long long*>(p);
   p += count_bytes;
-  b->d_cycles = reinterpret_cast<unsigned long long*>(p);
-  p += cyc_bytes;
   b->d_rootcnt = reinterpret_cast<unsigned int*>(p);
-  b->d_stride = b->d_rootcnt + nq + 1;
-  b->d_queue = b->d_stride + nq + 1;
+  b->d_queue = b->d_rootcnt + nq + 1;
   p += small_bytes;
   uint32_t* d_lvoff = reinterpret_cast<uint32_t*>(p);
   b->dg = DevGraph{g->off, g->ladj, g->loff, g->n_labels};
@@ -524,8 +547,8 @@ int prepare_batch(gpm_graph* g, const gpm_query* queries, int n_queries, const g
   int st;
   if ((st = h2d(g, d_lvoff, g->h_lvoff.data(), g->h_lvoff.size() * 4)) != GPM_OK) return st;
   if ((st = h2d(g, b->d_plans, plans.data(), (size_t)nq * sizeof(QPlan))) != GPM_OK) return st;
-  hipLaunchKernelGGL(zero_kernel, dim3((nq + 64) / 64 + 1), dim3(64), 0, g->stream, b->d_cycles,
-                     (int)nq, b->d_rootcnt, (int)nq);
+  hipLaunchKernelGGL(zero_kernel, dim3((nq + 64) / 64 + 1), dim3(64), 0, g->stream, b->d_counts,
+                     0, b->d_rootcnt, (int)nq);
   uint32_t maxlab = 0;
   for (auto& q : plans)
     maxlab = std::max(maxlab, g->h_lvoff[q.label[0] + 1] - g->h_lvoff[q.label[0]]);
@@ -538,37 +561,12 @@ int prepare_batch(gpm_graph* g, const gpm_query* queries, int n_queries, const g
   GPM_HIP(hipMemcpyAsync(rootcnt.data(), b->d_rootcnt, nq * 4, hipMemcpyDeviceToHost, g->stream));
   GPM_HIP(hipStreamSynchronize(g->stream));
   for (uint32_t i = 0; i < nq; ++i) plans[i].n_roots = rootcnt[i];
-  // Autotuner: a probe over a sample of every query's roots measures cycles
-  // per root.
+  // Autotuner: estimated work per root of every query.
   std::vector<double> cost_per_root(nq, 1.0);
-  std::vector<uint32_t> stride(nq, 1);
-  if (tun.autotune) {
-    const uint32_t kSample = 32;
-    uint32_t t = 0;
-    for (uint32_t i = 0; i < nq; ++i) {
-      stride[i] = std::max<uint32_t>(1, plans[i].n_roots / kSample);
-      plans[i].task_off = t;
-      t += plans[i].n_roots ? (plans[i].n_roots + stride[i] - 1) / stride[i] : 0;
-    }
-    const uint32_t probe_tasks = t;
-    if ((st = h2d(g, b->d_plans, plans.data(), (size_t)nq * sizeof(QPlan))) != GPM_OK) return st;
-    if ((st = h2d(g, b->d_stride, stride.data(), (size_t)nq * 4)) != GPM_OK) return st;
-    hipLaunchKernelGGL(zero_kernel, dim3((nq + 64) / 64 + 1), dim3(64), 0, g->stream, b->d_cycles,
-                       (int)nq, b->d_queue, 4);
-    hipLaunchKernelGGL(match_kernel<true>, dim3(b->grid), dim3(kBlock), 0, g->stream, b->dg,
-                       b->d_plans, nq, probe_tasks, b->d_roots, b->d_stride, b->d_queue,
-                       b->d_counts, b->d_cycles);
-    GPM_HIP(hipGetLastError());
-    std::vector<unsigned long long> cyc(nq);
-    GPM_HIP(hipMemcpyAsync(cyc.data(), b->d_cycles, nq * 8, hipMemcpyDeviceToHost, g->stream));
-    GPM_HIP(hipStreamSynchronize(g->stream));
-    for (uint32_t i = 0; i < nq; ++i) {
-      const uint32_t sampled = plans[i].n_roots ? (plans[i].n_roots + stride[i] - 1) / stride[i] : 0;
-      cost_per_root[i] = sampled ? std::max(1.0, (double)cyc[i] / sampled) : 1.0;
-    }
-  }
+  if (tun.autotune)
+    for (uint32_t i = 0; i < nq; ++i) cost_per_root[i] = estimate_root_cost(g, plans[i]);
   // Chunks (roots per task) and layout, heaviest queries first when tuned.
-  const double kTaskCycles = 2.0e5;
+  const double kTaskWork = 4096.0;  // estimated entries streamed per task
   std::vector<uint32_t> perm(nq);
   std::iota(perm.begin(), perm.end(), 0u);
   if (tun.autotune || tun.policy == GPM_SCHED_LEARNED)
@@ -583,7 +581,7 @@ int prepare_batch(gpm_graph* g, const gpm_query* queries, int n_queries, const g
     if (tun.chunk)
       q.chunk = tun.chunk;
     else if (tun.autotune)
-      q.chunk = (uint32_t)std::min(256.0, std::max(1.0, kTaskCycles / cost_per_root[i]));
+      q.chunk = (uint32_t)std::min(256.0, std::max(1.0, kTaskWork / cost_per_root[i]));
     else
       q.chunk = 8;
     q.task_off = t;
@@ -614,10 +612,17 @@ extern "C" int gpm_match_batch(gpm_graph* g, const gpm_query* queries, int n_que
   GPM_CHECK_ARG(g->labels && g->ladj, "matching needs a labeled graph");
   gpm_tuning tun{};
   tun.autotune = 1;
-  tun.use_graph = 1;
+  tun.use_graph = 0;
   if (tuning) tun = *tuning;
   GPM_CHECK_ARG(tun.policy == GPM_SCHED_DEGREE || tun.policy == GPM_SCHED_LEARNED,
                 "unknown policy %d", tun.policy);
+  // Autotuned batches are not captured into a hipGraph: that combination
+  // stopped responding on the device with the earlier probe-based tuner and
+  // stays disabled until its cause is known.  Tuned batches run as one
+  // persistent launch; graph replay is available with fixed chunks.
+  if (tun.autotune && tun.use_graph)
+    return fail(GPM_EUNSUPPORTED, "autotune with use_graph is not supported; "
+                                  "set autotune = 0 (fixed chunk) or use_graph = 0");
   for (int qi = 0; qi < n_queries; ++qi) {
     const gpm_query& gq = queries[qi];
     GPM_CHECK_ARG(gq.n >= 1 && gq.n <= (uint32_t)kMaxQ, "query %d: 1 <= n <= 16 (got %u)", qi,

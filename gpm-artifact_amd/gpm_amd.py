@@ -290,7 +290,7 @@ class DeviceGraph:
 
     def match_batch(self, queries: Sequence, autotune: int = 1,
                     policy: int = SCHED_DEGREE, chunk: int = 0,
-                    use_graph: int = 1) -> np.ndarray:
+                    use_graph: int = 0) -> np.ndarray:
         """queries: sequence of (n, edges [(u, v), ...], labels [..])."""
         arr = (Query * len(queries))()
         keep = []

@@ -250,6 +250,7 @@ typedef struct gpm_tuning {
  * (non-induced; automorphisms counted) -- the count the reference's CPU CECI
  * matcher prints as "#Embeddings" (sub_ceci/myceci.cpp:540-668, 1115).
  * All queries run in one persistent launch; per_query[i] is query i's count.
+ * tuning may be NULL: autotune = 1, policy DEGREE, chunk 0, use_graph = 0.
  */
 int gpm_match_batch(gpm_graph* g, const gpm_query* queries, int n_queries,
                     const gpm_tuning* tuning, uint64_t* per_query);

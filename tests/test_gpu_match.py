@@ -17,8 +17,15 @@ def hprd(gpm):
     dg.close()
 
 
+GRAPH_AUTOTUNE_OFF = pytest.mark.skip(
+    reason="autotune + hipGraph stopped responding twice on the device (device-probe tuner, "
+           "now removed); the combination is rejected by the library and not rerun")
+
+
 @pytest.mark.parametrize("autotune,policy,chunk,use_graph", [
-    (1, 0, 0, 1), (0, 0, 4, 0), (1, 1, 0, 1), (0, 1, 1, 1)])
+    pytest.param(1, 0, 0, 1, marks=GRAPH_AUTOTUNE_OFF), (0, 0, 4, 0),
+    pytest.param(1, 1, 0, 1, marks=GRAPH_AUTOTUNE_OFF), (0, 1, 1, 1),
+    (1, 0, 0, 0), (1, 1, 0, 0), (0, 0, 8, 1)])
 def test_hprd_known_answers(gpm, hprd, autotune, policy, chunk, use_graph):
     dg, qs, expected = hprd
     got = dg.match_batch(qs, autotune=autotune, policy=policy, chunk=chunk, use_graph=use_graph)
@@ -79,6 +86,10 @@ def test_unknown_label_and_errors(gpm):
                               (3, np.array([[0, 1], [1, 2], [0, 2]], dtype=np.uint32),
                                np.zeros(3, dtype=np.uint32))])
         assert got.tolist() == [0, 336]
+        # Autotuned batches are not replayed as a hipGraph: rejected up front.
+        with pytest.raises(gpm.GpmError):
+            dg.match_batch([(3, np.array([[0, 1], [1, 2], [0, 2]], dtype=np.uint32),
+                             np.zeros(3, dtype=np.uint32))], autotune=1, use_graph=1)
         with pytest.raises(gpm.GpmError):  # disconnected
             dg.match_batch([(3, np.array([[0, 1]], dtype=np.uint32), np.zeros(3, dtype=np.uint32))])
         with pytest.raises(gpm.GpmError):  # too many vertices
