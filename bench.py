@@ -81,6 +81,96 @@ def cpu_baseline(d, counts_dev, dg, budget_s):
     }
 
 
+def extra_tc(with_cpu):
+    """Triangle counting on seeded G(n=16384, M=131072), degree-oriented."""
+    g = G.er_graph(16384, 131072, 20240601)
+    d = G.orient(g)
+    out = {"workload": "TC on Erdos-Renyi n=16384, M=131072, degree-oriented DAG",
+           "oriented_edges": d.m}
+    with G.DeviceGraph(d, oriented=True) as dg:
+        plan = dg.clique_plan_bytes(3)
+        out["algorithmic_bytes"] = plan
+        for name, var in (("edge", G.TC_EDGE), ("vertex", G.TC_VERTEX)):
+            c = dg.tc(var)
+            single = dg.last_kernel_ms()
+            reps = 500
+            c2 = dg.tc_repeat(var, reps)
+            assert c2 == c
+            mean = dg.last_kernel_ms()
+            out[name] = {
+                "count": int(c), "single_launch_ms": single,
+                "mean_ms_back_to_back": mean, "reps": reps,
+                "edges_per_s": d.m / (mean * 1e-3),
+                "hbm_roofline_frac": plan / (mean * 1e-3) / (HBM_PEAK_GBS * 1e9),
+            }
+    if with_cpu:
+        from oracle import oracle as O
+        t0 = time.perf_counter()
+        want = O.tc(d.offsets, d.adj)
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": d.m / t, "unit": "edges/s", "cores": O.num_threads(),
+                               "kind": "port", "count": int(want),
+                               "exact_vs_engine": bool(want == out["edge"]["count"] ==
+                                                       out["vertex"]["count"])}
+    return out
+
+
+def extra_sched(scale):
+    """Learned vs plain degree scheduling: 5-clique on R-MAT scale 22, plus the
+    3-/4-vertex motif counts of the same graph."""
+    from cost_model import CostModel
+    g = G.rmat_graph(scale, 16, 20240522)
+    d = G.orient(g)
+    out = {"workload": f"5-clique + motifs on R-MAT scale {scale}, edge factor 16",
+           "n": d.n, "oriented_edges": d.m, "max_out_degree": int(d.degrees().max())}
+    model = CostModel.load_default(5)
+    with G.DeviceGraph(g, oriented=False) as sym, G.DeviceGraph(d, oriented=True) as dag:
+        t0 = time.perf_counter()
+        cost = model.predict(sym, dag)
+        out["model_seconds"] = time.perf_counter() - t0
+        plain = dag.clique(5)
+        learned = dag.clique(5, policy=G.SCHED_LEARNED, cost=cost)
+        t_plain, t_learned = [], []
+        for _ in range(2):
+            assert dag.clique(5) == plain
+            t_plain.append(dag.last_kernel_ms())
+            assert dag.clique(5, policy=G.SCHED_LEARNED, cost=cost) == learned
+            t_learned.append(dag.last_kernel_ms())
+        out["k5"] = {"count_plain": int(plain), "count_learned": int(learned),
+                     "identical": bool(plain == learned),
+                     "plain_ms": min(t_plain), "learned_ms": min(t_learned),
+                     "speedup": min(t_plain) / min(t_learned),
+                     "learned_breakdown_ms": dict(dag.kernel_breakdown())}
+        tri = dag.clique(3)
+        deg = g.degrees().astype(np.int64)
+        motif, _ = sym.pattern("Motif4", G.FORM_PATTERN)
+        out["motifs"] = {"triangles": int(tri),
+                         "wedges_induced": int((deg * (deg - 1) // 2).sum() - 3 * tri),
+                         "motif4_induced": dict(zip(["3-star", "4-path", "tailed-triangle",
+                                                     "4-cycle", "diamond", "4-clique"],
+                                                    [int(x) for x in motif])),
+                         "motif4_ms": sym.last_kernel_ms()}
+    return out
+
+
+def extra_match(scale, n_batch):
+    """Labeled matching: 5 named queries (scale 18) and a batch of sampled
+    queries (scale 21)."""
+    import workloads as W
+    out = {}
+    g, labels = W.labeled_graph(scale)
+    qs = [q for _, q in W.named_queries()]
+    with G.DeviceGraph(g, oriented=False, labels=labels, n_labels=W.N_LABELS) as dg:
+        res = {}
+        for pol, name in ((G.SCHED_DEGREE, "plain"), (G.SCHED_LEARNED, "learned")):
+            c = dg.match_batch(qs, autotune=1, policy=pol)
+            res[name] = {"counts": [int(x) for x in c], "ms": dg.last_kernel_ms()}
+        res["identical"] = res["plain"]["counts"] == res["learned"]["counts"]
+        res["embeddings_per_s"] = sum(res["learned"]["counts"]) / (res["learned"]["ms"] * 1e-3)
+        out["named"] = dict(workload=f"5 named queries on labeled R-MAT scale {scale}", **res)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +179,10 @@ def main():
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--extras", default="tc,sched",
+                    help="comma list of extra workloads: tc, sched, match (or none)")
+    ap.add_argument("--sched-scale", type=int, default=22)
+    ap.add_argument("--match-scale", type=int, default=18)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the "
@@ -178,6 +272,18 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(d, counts, dg, args.cpu_seconds)
     dg.close()
+    extras = [e for e in args.extras.split(",") if e and e != "none"]
+    out["extra"] = {}
+    for e in extras:
+        t0 = time.perf_counter()
+        if e == "tc":
+            out["extra"]["tc"] = extra_tc(not args.no_cpu)
+        elif e == "sched":
+            out["extra"]["sched"] = extra_sched(args.sched_scale)
+        elif e == "match":
+            out["extra"]["match"] = extra_match(args.match_scale, 256)
+        print(f"[bench] extra {e} done in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
+              flush=True)
     print(json.dumps(out), flush=True)
 
 

@@ -38,6 +38,12 @@ constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr uint32_t kWaveTable = 128;
 constexpr uint32_t kBlockCap = 512;
+// Threads per block of the block tiers.  Rows are shared by the whole block,
+// so wider blocks raise the waves per SIMD that hide LDS latency without
+// more LDS: the wide tier fits one block per CU, the LDS tier three.
+constexpr int kLdsBlock = 512;
+constexpr int kWideBlock = 1024;
+constexpr int kHbmBlock = 1024;
 enum { kCount = 0, kBytes = 1, kWork = 2 };
 
 __device__ __forceinline__ bool use_probe(uint32_t len, uint32_t d) {
@@ -223,66 +229,119 @@ __device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t* s_wsum
 
 // Per-root counting (or accounting) over W-word rows; thread-owned rows.
 // K=4/5 counting with the (i, j in row_i) pairs dealt out evenly over the
-// block: rows are taken kBlock at a time, a block scan of their popcounts
+// block: rows are taken BS at a time, a block scan of their popcounts
 // gives each pair an owner row, and j is found by popcount over the words.
-// MAXW > 0 keeps row_i & row_j in registers (W <= MAXW).
-template <int K, int MAXW>
+//
+// MAXW > 0 (rows in LDS, W <= MAXW): rows are read MAXW words at a time
+// with no per-word branch, so all loads of a row are in flight together and
+// one wait covers them (words past W belong to the next row or to the tail
+// of the block's LDS, and are masked out by m[x] = 0 for x >= W).  For K=5
+// the rows l of row_i & row_j are taken two at a time for the same reason.
+// MAXW == 0 is the generic path for rows in HBM.
+template <int K, int MAXW, int BS>
 __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, uint32_t d,
                                                 uint32_t W, uint32_t part, uint32_t n_parts,
                                                 uint32_t* s_pre, uint32_t* s_wsum) {
   uint64_t c = 0;
   // This task counts rows part, part + n_parts, ...: nr of them.
   const uint32_t nr = d > part ? (d - part + n_parts - 1) / n_parts : 0u;
-  for (uint32_t c0 = 0; c0 < nr; c0 += kBlock) {
+  for (uint32_t c0 = 0; c0 < nr; c0 += BS) {
     const uint32_t t0 = c0 + threadIdx.x;
     uint32_t pc = 0;
     if (t0 < nr) {
-      const uint32_t i0 = part + t0 * n_parts;
-      for (uint32_t w = 0; w < W; ++w) pc += (uint32_t)__popcll(rows[(size_t)i0 * W + w]);
+      const unsigned long long* r0 = rows + (size_t)(part + t0 * n_parts) * W;
+      if (MAXW > 0) {
+#pragma unroll
+        for (int w = 0; w < MAXW; ++w) {
+          const unsigned long long x = r0[w];
+          pc += w < (int)W ? (uint32_t)__popcll(x) : 0u;
+        }
+      } else {
+        for (uint32_t w = 0; w < W; ++w) pc += (uint32_t)__popcll(r0[w]);
+      }
     }
     const uint32_t incl = block_incl_scan(pc, s_wsum);
     s_pre[threadIdx.x] = incl;
     __syncthreads();
-    const uint32_t total = uniform(s_pre[kBlock - 1]);
-    for (uint32_t p = threadIdx.x; p < total; p += kBlock) {
-      const uint32_t jj = upper_index(s_pre, kBlock, p);
+    const uint32_t total = uniform(s_pre[BS - 1]);
+    for (uint32_t p = threadIdx.x; p < total; p += BS) {
+      const uint32_t jj = upper_index(s_pre, BS, p);
       uint32_t kth = p - (jj ? s_pre[jj - 1] : 0u);
       const unsigned long long* ri = rows + (size_t)(part + (c0 + jj) * n_parts) * W;
-      uint32_t w = 0;
-      while (true) {
-        const uint32_t pw = (uint32_t)__popcll(ri[w]);
-        if (kth < pw) break;
-        kth -= pw;
-        ++w;
-      }
-      const uint32_t j = w * 64 + select_bit(ri[w], kth);
-      const unsigned long long* rj = rows + (size_t)j * W;
-      if (K == 4) {
-        for (uint32_t x = 0; x < W; ++x) c += __popcll(ri[x] & rj[x]);
-      } else if (MAXW > 0) {
-        uint64_t m[MAXW > 0 ? MAXW : 1];
+      if (MAXW > 0) {
+        uint64_t wi[MAXW > 0 ? MAXW : 1];
 #pragma unroll
-        for (int x = 0; x < MAXW; ++x) m[x] = (x < (int)W) ? (ri[x] & rj[x]) : 0ull;
+        for (int x = 0; x < MAXW; ++x) wi[x] = x < (int)W ? ri[x] : 0ull;
+        // j = position of the kth set bit of row i.
+        uint32_t j = 0;
+        bool found = false;
 #pragma unroll
         for (int x = 0; x < MAXW; ++x) {
-          uint64_t mm = m[x];
-          while (mm) {
-            const uint32_t l = x * 64 + ctz64(mm);
-            mm &= mm - 1;
-            const unsigned long long* rl = rows + (size_t)l * W;
+          const uint32_t pw = (uint32_t)__popcll(wi[x]);
+          if (!found && kth < pw) {
+            j = x * 64 + select_bit(wi[x], kth);
+            found = true;
+          } else if (!found) {
+            kth -= pw;
+          }
+        }
+        const unsigned long long* rj = rows + (size_t)j * W;
+        uint64_t m[MAXW > 0 ? MAXW : 1];
 #pragma unroll
-            for (int y = 0; y < MAXW; ++y)
-              if (y < (int)W) c += __popcll(m[y] & rl[y]);
+        for (int x = 0; x < MAXW; ++x) m[x] = wi[x] & rj[x];
+        if (K == 4) {
+#pragma unroll
+          for (int x = 0; x < MAXW; ++x) c += __popcll(m[x]);
+        } else {
+#pragma unroll
+          for (int x = 0; x < MAXW; ++x) {
+            uint64_t mm = m[x];
+            while (mm) {
+              const uint32_t l1 = x * 64 + ctz64(mm);
+              mm &= mm - 1;
+              const unsigned long long* r1 = rows + (size_t)l1 * W;
+              if (MAXW <= 8 && mm) {
+                const uint32_t l2 = x * 64 + ctz64(mm);
+                mm &= mm - 1;
+                const unsigned long long* r2 = rows + (size_t)l2 * W;
+                uint64_t a[MAXW > 0 ? MAXW : 1], b[MAXW > 0 ? MAXW : 1];
+#pragma unroll
+                for (int y = 0; y < MAXW; ++y) a[y] = r1[y];
+#pragma unroll
+                for (int y = 0; y < MAXW; ++y) b[y] = r2[y];
+#pragma unroll
+                for (int y = 0; y < MAXW; ++y) c += __popcll(m[y] & a[y]) + __popcll(m[y] & b[y]);
+              } else {
+                uint64_t a[MAXW > 0 ? MAXW : 1];
+#pragma unroll
+                for (int y = 0; y < MAXW; ++y) a[y] = r1[y];
+#pragma unroll
+                for (int y = 0; y < MAXW; ++y) c += __popcll(m[y] & a[y]);
+              }
+            }
           }
         }
       } else {
-        for (uint32_t x = 0; x < W; ++x) {
-          uint64_t mm = ri[x] & rj[x];
-          while (mm) {
-            const uint32_t l = x * 64 + ctz64(mm);
-            mm &= mm - 1;
-            const unsigned long long* rl = rows + (size_t)l * W;
-            for (uint32_t y = 0; y < W; ++y) c += __popcll(ri[y] & rj[y] & rl[y]);
+        uint32_t w = 0;
+        while (true) {
+          const uint32_t pw = (uint32_t)__popcll(ri[w]);
+          if (kth < pw) break;
+          kth -= pw;
+          ++w;
+        }
+        const uint32_t j = w * 64 + select_bit(ri[w], kth);
+        const unsigned long long* rj = rows + (size_t)j * W;
+        if (K == 4) {
+          for (uint32_t x = 0; x < W; ++x) c += __popcll(ri[x] & rj[x]);
+        } else {
+          for (uint32_t x = 0; x < W; ++x) {
+            uint64_t mm = ri[x] & rj[x];
+            while (mm) {
+              const uint32_t l = x * 64 + ctz64(mm);
+              mm &= mm - 1;
+              const unsigned long long* rl = rows + (size_t)l * W;
+              for (uint32_t y = 0; y < W; ++y) c += __popcll(ri[y] & rj[y] & rl[y]);
+            }
           }
         }
       }
@@ -292,12 +351,12 @@ __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, 
   return c;
 }
 
-template <int K, int MODE>
+template <int K, int MODE, int BS>
 __device__ __forceinline__ uint64_t count_rows(const unsigned long long* rows, uint32_t d,
                                                uint32_t W, uint32_t part, uint32_t n_parts,
                                                const uint32_t* lens) {
   uint64_t c = 0;
-  for (uint32_t i = part + threadIdx.x * n_parts; i < d; i += kBlock * n_parts) {
+  for (uint32_t i = part + threadIdx.x * n_parts; i < d; i += BS * n_parts) {
     const unsigned long long* ri = rows + (size_t)i * W;
     if (MODE == kCount) {
       for (uint32_t w = 0; w < W; ++w) c += __popcll(ri[w]);  // K == 3 only
@@ -339,19 +398,19 @@ __device__ __forceinline__ uint64_t count_rows(const unsigned long long* rows, u
 // Block per root.  LDS_ROWS: rows, S, lens, begins and the hash table live in
 // dynamic LDS (|S| <= cap, W <= MAXW words per row).  Otherwise rows live in this block's
 // HBM region `grows` (cap*W words) and membership in S is a binary search.
-template <int K, int MODE, bool LDS_ROWS, int MAXW>
-__global__ __launch_bounds__(kBlock) void clique_block_kernel(
+template <int K, int MODE, bool LDS_ROWS, int MAXW, int BS>
+__global__ __launch_bounds__(BS) void clique_block_kernel(
     const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
     const uint32_t* __restrict__ roots, const uint32_t* __restrict__ parts, uint32_t n_roots,
     uint32_t cap, uint32_t table, unsigned int* __restrict__ queue,
     unsigned long long* __restrict__ grows, uint32_t* __restrict__ gmeta,
     unsigned long long* __restrict__ out, unsigned long long* __restrict__ work) {
   extern __shared__ unsigned long long s_dyn[];
-  __shared__ uint32_t s_pre[kBlock];
-  __shared__ uint32_t s_wsum[kWavesPerBlock];
+  __shared__ uint32_t s_pre[BS];
+  __shared__ uint32_t s_wsum[(BS / kWave)];
   __shared__ uint32_t s_root;
   __shared__ int s_any_probe;
-  __shared__ unsigned long long s_red[kWavesPerBlock];
+  __shared__ unsigned long long s_red[(BS / kWave)];
   const uint32_t Wcap = (cap + 63) / 64;
   unsigned long long* rows;
   uint32_t *lens, *begs, *keys;
@@ -384,25 +443,25 @@ __global__ __launch_bounds__(kBlock) void clique_block_kernel(
     const uint32_t* S = adj + sb;
     const uint32_t W = (d + 63) / 64;
     if (threadIdx.x == 0) s_any_probe = 0;
-    for (uint32_t i = threadIdx.x; i < d; i += kBlock) {
+    for (uint32_t i = threadIdx.x; i < d; i += BS) {
       const uint32_t v = S[i];
       const uint32_t b = off[v];
       const uint32_t len = off[v + 1] - b;
       lens[i] = len;
       begs[i] = b;
     }
-    for (uint32_t i = threadIdx.x; i < d * W; i += kBlock) rows[i] = 0ull;
+    for (uint32_t i = threadIdx.x; i < d * W; i += BS) rows[i] = 0ull;
     if (LDS_ROWS)
-      for (uint32_t i = threadIdx.x; i < table; i += kBlock) keys[i] = kEmpty;
+      for (uint32_t i = threadIdx.x; i < table; i += BS) keys[i] = kEmpty;
     __syncthreads();
     // HBM rows: drop stale L1 lines of the previous root before reading.
     if (!LDS_ROWS) __threadfence();
     if (LDS_ROWS)
-      for (uint32_t i = threadIdx.x; i < d; i += kBlock)
+      for (uint32_t i = threadIdx.x; i < d; i += BS)
         lds_hash_insert(keys, vals, hmask, S[i], i);
     __syncthreads();
     // Flattened walk over the lists that are streamed.
-    for (uint32_t c0 = 0; c0 < d; c0 += kBlock) {
+    for (uint32_t c0 = 0; c0 < d; c0 += BS) {
       const uint32_t i = c0 + threadIdx.x;
       uint32_t wl = 0;
       if (i < d) {
@@ -415,9 +474,9 @@ __global__ __launch_bounds__(kBlock) void clique_block_kernel(
       const uint32_t incl = block_incl_scan(wl, s_wsum);
       s_pre[threadIdx.x] = incl;
       __syncthreads();
-      const uint32_t total = uniform(s_pre[kBlock - 1]);
-      for (uint32_t f = threadIdx.x; f < total; f += kBlock) {
-        const uint32_t j = upper_index(s_pre, kBlock, f);
+      const uint32_t total = uniform(s_pre[BS - 1]);
+      for (uint32_t f = threadIdx.x; f < total; f += BS) {
+        const uint32_t j = upper_index(s_pre, BS, f);
         const uint32_t excl = j ? s_pre[j - 1] : 0u;
         const uint32_t row_i = c0 + j;
         const uint32_t x = adj[begs[row_i] + (f - excl)];
@@ -439,7 +498,7 @@ __global__ __launch_bounds__(kBlock) void clique_block_kernel(
         const uint32_t len = lens[i];
         if (!use_probe(len, d)) continue;
         const uint32_t* Ni = adj + begs[i];
-        for (uint32_t t0 = 0; t0 < d; t0 += kBlock) {
+        for (uint32_t t0 = 0; t0 < d; t0 += BS) {
           const uint32_t t = t0 + threadIdx.x;
           const bool found = t < d && sorted_contains(Ni, len, S[t]);
           const uint64_t m = wave_ballot(found);
@@ -450,16 +509,27 @@ __global__ __launch_bounds__(kBlock) void clique_block_kernel(
     }
     __syncthreads();
     if (!LDS_ROWS) __threadfence();
-    uint64_t c = (MODE == kCount && K >= 4)
-                     ? count_pairs<K, MAXW>(rows, d, W, part, n_parts, s_pre, s_wsum)
-                     : count_rows<K, MODE>(rows, d, W, part, n_parts, lens);
+    uint64_t c;
+    if (MODE == kCount && K >= 4) {
+      // Narrowest register width that holds this root's rows.
+      if (MAXW == 8 && W <= 2)
+        c = count_pairs<K, (MAXW == 8 ? 2 : MAXW), BS>(rows, d, W, part, n_parts, s_pre, s_wsum);
+      else if (MAXW == 8 && W <= 4)
+        c = count_pairs<K, (MAXW == 8 ? 4 : MAXW), BS>(rows, d, W, part, n_parts, s_pre, s_wsum);
+      else if (MAXW == 16 && W <= 12)
+        c = count_pairs<K, (MAXW == 16 ? 12 : MAXW), BS>(rows, d, W, part, n_parts, s_pre, s_wsum);
+      else
+        c = count_pairs<K, MAXW, BS>(rows, d, W, part, n_parts, s_pre, s_wsum);
+    } else {
+      c = count_rows<K, MODE, BS>(rows, d, W, part, n_parts, lens);
+    }
     if (MODE == kWork) {
       c = wave_sum_u64(c);
       if (lane_id() == 0) s_red[threadIdx.x / kWave] = c;
       __syncthreads();
       if (threadIdx.x == 0) {
         unsigned long long t = 0;
-        for (int i = 0; i < kWavesPerBlock; ++i) t += s_red[i];
+        for (int i = 0; i < (BS / kWave); ++i) t += s_red[i];
         work[u] = t / 4;
       }
     } else {
@@ -512,14 +582,16 @@ uint32_t lds_wide_cap() {
   return (uint32_t)cached;
 }
 
-// Blocks of an LDS-rows tier of this cap launched per CU (LDS-limited, <= 8).
-int lds_blocks_per_cu(uint32_t cap) {
+// Blocks of an LDS-rows tier of this cap resident per CU: limited by LDS
+// (160 KiB per CU) and by 32 waves per CU.
+int lds_blocks_per_cu(uint32_t cap, int block_threads) {
   const size_t lds = block_lds_bytes(cap, table_for(cap));
-  return std::min(8, std::max<int>(1, (int)(160 * 1024 / (lds + 2048))));
+  const int by_waves = std::max(1, 32 / (block_threads / kWave));
+  return std::min(by_waves, std::max<int>(1, (int)(160 * 1024 / (lds + 2048))));
 }
 
 // One LDS-rows tier: rows of up to MAXW words, cap = its largest out-degree.
-template <int K, int MODE, int MAXW>
+template <int K, int MODE, int MAXW, int BS>
 int launch_lds_tier(gpm_graph* g, const uint32_t* roots, const uint32_t* parts,
                     uint32_t n_roots, uint32_t cap,
                     unsigned int* queue, unsigned long long* out, unsigned long long* work,
@@ -527,13 +599,13 @@ int launch_lds_tier(gpm_graph* g, const uint32_t* roots, const uint32_t* parts,
   const int cus = device_cus(g);
   const uint32_t table = table_for(cap);
   const size_t lds = block_lds_bytes(cap, table);
-  auto kern = clique_block_kernel<K, MODE, true, MAXW>;
+  auto kern = clique_block_kernel<K, MODE, true, MAXW, BS>;
   if (lds > 48 * 1024)
     GPM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const unsigned blocks =
-      (unsigned)std::min<uint64_t>(n_roots, (uint64_t)cus * lds_blocks_per_cu(cap));
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), lds, g->stream, g->off, g->adj, roots,
+      (unsigned)std::min<uint64_t>(n_roots, (uint64_t)cus * lds_blocks_per_cu(cap, BS));
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(BS), lds, g->stream, g->off, g->adj, roots,
                      parts, n_roots, cap, table, queue, nullptr, nullptr, out, work);
   GPM_HIP(hipGetLastError());
   if (MODE == kCount) timing_mark(g, name);
@@ -557,7 +629,8 @@ int launch_tiers(gpm_graph* g, const RootTiers* t, unsigned long long* out,
     if (st != GPM_OK) return st;
     unsigned long long* grows = static_cast<unsigned long long*>(g->scratch);
     uint32_t* gmeta = reinterpret_cast<uint32_t*>(static_cast<char*>(g->scratch) + row_bytes);
-    hipLaunchKernelGGL((clique_block_kernel<K, MODE, false, 0>), dim3(blocks), dim3(kBlock), 0,
+    hipLaunchKernelGGL((clique_block_kernel<K, MODE, false, 0, kHbmBlock>), dim3(blocks),
+                       dim3(kHbmBlock), 0,
                        g->stream, g->off, g->adj, t->d_big, t->d_big_parts, t->n_big, cap, 0u,
                        q + 2, grows,
                        gmeta, out, work);
@@ -565,12 +638,12 @@ int launch_tiers(gpm_graph* g, const RootTiers* t, unsigned long long* out,
     if (MODE == kCount) timing_mark(g, "clique_hbm_block");
   }
   if (t->n_block) {
-    int st = launch_lds_tier<K, MODE, 8>(g, t->d_block, t->d_block_parts, t->n_block,
+    int st = launch_lds_tier<K, MODE, 8, kLdsBlock>(g, t->d_block, t->d_block_parts, t->n_block,
                                          t->max_block_deg, q + 1, out, work, "clique_lds_block");
     if (st != GPM_OK) return st;
   }
   if (t->n_block2) {
-    int st = launch_lds_tier<K, MODE, 16>(g, t->d_block2, t->d_block2_parts, t->n_block2,
+    int st = launch_lds_tier<K, MODE, 16, kWideBlock>(g, t->d_block2, t->d_block2_parts, t->n_block2,
                                           t->max_block2_deg, q + 3, out, work,
                                           "clique_lds_block_wide");
     if (st != GPM_OK) return st;
@@ -640,8 +713,8 @@ int gpm_clique(gpm_graph* g, int k, const gpm_sched* sched, uint64_t* out) {
   if (req.policy == GPM_SCHED_LEARNED) {
     req.heavy_cost = learned_heavy_cost(req.cost, g->n);
     const uint32_t cus = (uint32_t)device_cus(g);
-    req.resident_block = cus * lds_blocks_per_cu(req.block_cap);
-    req.resident_block2 = cus * lds_blocks_per_cu(req.block2_cap);
+    req.resident_block = cus * lds_blocks_per_cu(req.block_cap, kLdsBlock);
+    req.resident_block2 = cus * lds_blocks_per_cu(req.block2_cap, kWideBlock);
     req.resident_big = cus;
   }
   const RootTiers* t = nullptr;

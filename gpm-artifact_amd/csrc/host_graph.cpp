@@ -300,6 +300,78 @@ int gpm_load_mtx(const char* path, uint32_t* n, uint32_t** src, uint32_t** dst,
   return GPM_OK;
 }
 
+int gpm_load_labeled(const char* path, uint32_t* n, uint32_t** offsets, uint32_t** adj,
+                     uint64_t* m, uint32_t** labels, uint32_t* n_labels) {
+  clear_error();
+  GPM_CHECK_ARG(path && n && offsets && adj && m && labels && n_labels, "null pointer");
+  std::ifstream fin(path);
+  if (!fin.is_open()) return fail(GPM_EIO, "cannot open file %s", path);
+  // Header "t |V| |E|" (sub_ceci/graph/graph.cpp:94-95).
+  char type = 0;
+  int64_t nv = -1, ne = -1;
+  if (!(fin >> type >> nv >> ne) || type != 't' || nv < 0 || ne < 0 || nv > 0xFFFFFFFEll ||
+      2 * ne >= (1ll << 32))
+    return fail(GPM_EIO, "%s: bad header (want \"t |V| |E|\")", path);
+  std::vector<uint32_t> off(nv + 1, 0), lab(nv, 0), fill(nv, 0);
+  std::vector<uint32_t> nb((size_t)2 * ne);
+  int64_t next_v = 0, n_e = 0;
+  uint32_t max_label = 0;
+  // "v id label degree" lines in id order, then "e u v" lines, each
+  // undirected edge once (graph.cpp:106-144).  The declared degrees size the
+  // lists, as in the reference; they must add up and be met exactly.
+  while (fin >> type) {
+    if (type == 'v') {
+      int64_t id = -1, l = -1, deg = -1;
+      if (!(fin >> id >> l >> deg) || id != next_v || l < 0 || l > 0xFFFFFFFEll || deg < 0)
+        return fail(GPM_EIO, "%s: bad vertex line for vertex %lld", path, (long long)next_v);
+      if ((int64_t)off[id] + deg > 2 * ne)
+        return fail(GPM_EIO, "%s: declared degrees exceed 2|E|", path);
+      lab[id] = (uint32_t)l;
+      max_label = std::max(max_label, (uint32_t)l);
+      off[id + 1] = off[id] + (uint32_t)deg;
+      ++next_v;
+    } else if (type == 'e') {
+      int64_t a = -1, b = -1;
+      if (!(fin >> a >> b) || a < 0 || b < 0 || a >= next_v || b >= next_v || a == b)
+        return fail(GPM_EIO, "%s: bad edge line %lld", path, (long long)n_e);
+      if (off[a] + fill[a] >= off[a + 1] || off[b] + fill[b] >= off[b + 1])
+        return fail(GPM_EIO, "%s: edge %lld exceeds a declared degree", path, (long long)n_e);
+      nb[off[a] + fill[a]++] = (uint32_t)b;
+      nb[off[b] + fill[b]++] = (uint32_t)a;
+      ++n_e;
+    } else {
+      return fail(GPM_EIO, "%s: unknown line type '%c'", path, type);
+    }
+  }
+  if (next_v != nv || n_e != ne || off[nv] != 2 * ne)
+    return fail(GPM_EIO, "%s: header says %lld vertices / %lld edges, file has %lld / %lld",
+                path, (long long)nv, (long long)ne, (long long)next_v, (long long)n_e);
+  for (int64_t v = 0; v < nv; ++v) {
+    if (fill[v] != off[v + 1] - off[v])
+      return fail(GPM_EIO, "%s: vertex %lld has fewer edges than declared", path, (long long)v);
+    std::sort(nb.begin() + off[v], nb.begin() + off[v + 1]);  // graph.cpp:156-158
+  }
+  uint32_t* o = xmalloc<uint32_t>((size_t)nv + 1);
+  uint32_t* a = xmalloc<uint32_t>(std::max<size_t>(nb.size(), 1));
+  uint32_t* l = xmalloc<uint32_t>(std::max<int64_t>(nv, 1));
+  if (!o || !a || !l) {
+    std::free(o);
+    std::free(a);
+    std::free(l);
+    return fail(GPM_ENOMEM, "host allocation failed");
+  }
+  std::copy(off.begin(), off.end(), o);
+  std::copy(nb.begin(), nb.end(), a);
+  std::copy(lab.begin(), lab.end(), l);
+  *n = (uint32_t)nv;
+  *offsets = o;
+  *adj = a;
+  *m = nb.size();
+  *labels = l;
+  *n_labels = nv ? max_label + 1 : 0;
+  return GPM_OK;
+}
+
 int gpm_gen_er(uint32_t n, uint64_t m_undirected, uint64_t seed,
                uint32_t** src, uint32_t** dst, uint64_t* n_edges) {
   clear_error();

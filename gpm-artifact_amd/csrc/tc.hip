@@ -206,19 +206,12 @@ int edge_group_size(const gpm_graph* g) {
   return 64;
 }
 
-}  // namespace
-
-extern "C" int gpm_tc(gpm_graph* g, int variant, uint64_t* out) {
-  clear_error();
-  GPM_CHECK_ARG(g && out, "null pointer");
-  GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "triangle counting needs an ORIENTED graph");
-  GPM_CHECK_ARG(variant == GPM_TC_EDGE || variant == GPM_TC_VERTEX, "unknown TC variant %d",
-                variant);
+// Enqueues one triangle count (counter reset + kernels) on g's stream; the
+// count lands in g->counters[0].  mark: record per-kernel timing marks.
+int enqueue_tc(gpm_graph* g, int variant, bool mark) {
   int st;
   const int cus = device_cus(g);
   if (variant == GPM_TC_EDGE) {
-    if ((st = ensure_src(g)) != GPM_OK) return st;
-    if ((st = timing_begin(g)) != GPM_OK) return st;
     if ((st = zero_counters(g, 1)) != GPM_OK) return st;
     const int G = edge_group_size(g);
     const uint64_t groups_wanted = std::max<uint64_t>(g->m, 1);
@@ -233,41 +226,75 @@ extern "C" int gpm_tc(gpm_graph* g, int variant, uint64_t* out) {
       default: hipLaunchKernelGGL(tc_edge_kernel<64>, grid, block, 0, g->stream, g->off, g->adj, g->src, g->m, g->counters); break;
     }
     GPM_HIP(hipGetLastError());
-    timing_mark(g, "tc_edge");
-  } else {
-    const RootTiers* t = nullptr;
-    if ((st = get_tc_tiers(g, kTcWaveCap, &t)) != GPM_OK) return st;
-    if ((st = timing_begin(g)) != GPM_OK) return st;
-    if ((st = zero_counters(g, 1)) != GPM_OK) return st;
-    unsigned int* q = t->queues;
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, g->stream, q, 4);
-    if (t->n_wave) {
-      const uint32_t chunk = 8;
-      const uint64_t waves = std::min<uint64_t>((t->n_wave + chunk - 1) / chunk, (uint64_t)cus * 32);
-      const unsigned blocks = (unsigned)std::max<uint64_t>(1, (waves * kWave + kBlock - 1) / kBlock);
-      hipLaunchKernelGGL(tc_vertex_wave_kernel, dim3(blocks), dim3(kBlock), 0, g->stream, g->off,
-                         g->adj, t->d_wave, t->n_wave, chunk, q, g->counters);
-      GPM_HIP(hipGetLastError());
-      timing_mark(g, "tc_vertex_wave");
-    }
-    if (t->n_block) {
-      uint32_t table = 0;
-      if (2ull * t->max_block_deg <= kTcBlockTableMax) {
-        table = 1;
-        while (table < 2 * t->max_block_deg) table <<= 1;
-      }
-      const unsigned blocks = (unsigned)std::min<uint64_t>(t->n_block, (uint64_t)cus * 4);
-      hipLaunchKernelGGL(tc_vertex_block_kernel, dim3(blocks), dim3(kBlock),
-                         table * sizeof(uint32_t), g->stream, g->off, g->adj, t->d_block,
-                         t->n_block, table, q + 1, g->counters);
-      GPM_HIP(hipGetLastError());
-      timing_mark(g, "tc_vertex_block");
-    }
+    if (mark) timing_mark(g, "tc_edge");
+    return GPM_OK;
   }
+  const RootTiers* t = nullptr;
+  if ((st = get_tc_tiers(g, kTcWaveCap, &t)) != GPM_OK) return st;
+  if ((st = zero_counters(g, 1)) != GPM_OK) return st;
+  unsigned int* q = t->queues;
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, g->stream, q, 4);
+  if (t->n_wave) {
+    const uint32_t chunk = 8;
+    const uint64_t waves = std::min<uint64_t>((t->n_wave + chunk - 1) / chunk, (uint64_t)cus * 32);
+    const unsigned blocks = (unsigned)std::max<uint64_t>(1, (waves * kWave + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(tc_vertex_wave_kernel, dim3(blocks), dim3(kBlock), 0, g->stream, g->off,
+                       g->adj, t->d_wave, t->n_wave, chunk, q, g->counters);
+    GPM_HIP(hipGetLastError());
+    if (mark) timing_mark(g, "tc_vertex_wave");
+  }
+  if (t->n_block) {
+    uint32_t table = 0;
+    if (2ull * t->max_block_deg <= kTcBlockTableMax) {
+      table = 1;
+      while (table < 2 * t->max_block_deg) table <<= 1;
+    }
+    const unsigned blocks = (unsigned)std::min<uint64_t>(t->n_block, (uint64_t)cus * 4);
+    hipLaunchKernelGGL(tc_vertex_block_kernel, dim3(blocks), dim3(kBlock),
+                       table * sizeof(uint32_t), g->stream, g->off, g->adj, t->d_block,
+                       t->n_block, table, q + 1, g->counters);
+    GPM_HIP(hipGetLastError());
+    if (mark) timing_mark(g, "tc_vertex_block");
+  }
+  return GPM_OK;
+}
+
+int tc_run(gpm_graph* g, int variant, int reps, uint64_t* out) {
+  GPM_CHECK_ARG(g && out, "null pointer");
+  GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "triangle counting needs an ORIENTED graph");
+  GPM_CHECK_ARG(variant == GPM_TC_EDGE || variant == GPM_TC_VERTEX, "unknown TC variant %d",
+                variant);
+  GPM_CHECK_ARG(reps >= 1, "reps must be >= 1 (got %d)", reps);
+  int st;
+  if (variant == GPM_TC_EDGE && (st = ensure_src(g)) != GPM_OK) return st;
+  if (variant == GPM_TC_VERTEX) {
+    const RootTiers* t = nullptr;  // build (and cache) the tiers before timing
+    if ((st = get_tc_tiers(g, kTcWaveCap, &t)) != GPM_OK) return st;
+  }
+  if ((st = timing_begin(g)) != GPM_OK) return st;
+  for (int r = 0; r < reps; ++r)
+    if ((st = enqueue_tc(g, variant, r == 0)) != GPM_OK) return st;
   if ((st = timing_end(g)) != GPM_OK) return st;
+  g->last_ms /= reps;
   unsigned long long h = 0;
   GPM_HIP(hipMemcpyAsync(&h, g->counters, sizeof(h), hipMemcpyDeviceToHost, g->stream));
   GPM_HIP(hipStreamSynchronize(g->stream));
   *out = h;
   return GPM_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+int gpm_tc(gpm_graph* g, int variant, uint64_t* out) {
+  clear_error();
+  return tc_run(g, variant, 1, out);
+}
+
+int gpm_tc_repeat(gpm_graph* g, int variant, int reps, uint64_t* out) {
+  clear_error();
+  return tc_run(g, variant, reps, out);
+}
+
+}  // extern "C"

@@ -146,6 +146,17 @@ def load_mtx(path: str):
     return n.value, _take(ps, ne.value), _take(pd, ne.value)
 
 
+def load_labeled(path: str):
+    """sub_ceci text graph -> (HostCSR symmetric sorted, labels, n_labels)."""
+    n, po, pa, m, pl, nl = C.c_uint32(), _u32p(), _u32p(), C.c_uint64(), _u32p(), C.c_uint32()
+    _check(lib().gpm_load_labeled(path.encode(), C.byref(n), C.byref(po), C.byref(pa),
+                                  C.byref(m), C.byref(pl), C.byref(nl)))
+    off = _take(po, n.value + 1)
+    adj = _take(pa, m.value)
+    lab = _take(pl, n.value)
+    return HostCSR(off, adj), lab, nl.value
+
+
 def gen_er(n: int, m: int, seed: int):
     ps, pd, ne = _u32p(), _u32p(), C.c_uint64()
     _check(lib().gpm_gen_er(C.c_uint32(n), C.c_uint64(m), C.c_uint64(seed),
@@ -250,6 +261,12 @@ class DeviceGraph:
     def tc(self, variant: int = TC_VERTEX) -> int:
         out = C.c_uint64()
         _check(lib().gpm_tc(self._h, C.c_int(variant), C.byref(out)))
+        return out.value
+
+    def tc_repeat(self, variant: int, reps: int) -> int:
+        """reps back-to-back counts; last_kernel_ms() is the mean per count."""
+        out = C.c_uint64()
+        _check(lib().gpm_tc_repeat(self._h, C.c_int(variant), C.c_int(reps), C.byref(out)))
         return out.value
 
     def clique(self, k: int, policy: int = SCHED_DEGREE, chunk: int = 0,

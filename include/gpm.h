@@ -91,6 +91,18 @@ int gpm_load_mtx(const char* path, uint32_t* n, uint32_t** src, uint32_t** dst,
                  uint64_t* n_edges);
 
 /*
+ * Labeled graph reader for the sub_ceci text format
+ * (sub_ceci/graph/graph.cpp:86-166): "t |V| |E|", then "v id label degree"
+ * lines in id order, then "e u v" lines with each undirected edge once.
+ * Returns the symmetric CSR with sorted lists and the labels; *n_labels =
+ * max label + 1.  Unlike the reference, inconsistent files (ids out of order,
+ * declared degrees not met, counts not matching the header) are rejected
+ * with GPM_EIO instead of being read past.  Query graphs use the same format.
+ */
+int gpm_load_labeled(const char* path, uint32_t* n, uint32_t** offsets, uint32_t** adj,
+                     uint64_t* m, uint32_t** labels, uint32_t* n_labels);
+
+/*
  * Seeded Erdos-Renyi G(n, M): M distinct undirected edges, no self loops,
  * emitted once each as (min, max).  No counterpart in the reference (which
  * only reads files); used for the triangle-counting workload.
@@ -179,6 +191,15 @@ enum gpm_tc_variant {
  * (src/TCSolver.cuh:10-121; e_centric :27-59, v_centric :64-113).
  */
 int gpm_tc(gpm_graph* g, int variant, uint64_t* out);
+
+/*
+ * gpm_tc repeated reps times back to back on the device (no host round trip
+ * between repetitions), for throughput measurement of launches too short to
+ * time one by one; *out is the count (every repetition recounts it) and
+ * gpm_last_kernel_ms reports the mean time per repetition.  No counterpart
+ * in the reference.
+ */
+int gpm_tc_repeat(gpm_graph* g, int variant, int reps, uint64_t* out);
 
 /*
  * k-clique count (3 <= k <= 5) of an ORIENTED graph, replacing CFSolver

@@ -102,3 +102,51 @@ def test_rmat_skew(gpm):
     deg = g.degrees()
     assert deg[0] == deg.max()
     assert np.all(np.diff(g.offsets.astype(np.int64)) >= 0)
+
+
+# ---------------------------------------------------------------- sub_ceci text format
+
+def _write_tve(path, g, labels):
+    deg = g.degrees()
+    with open(path, "w") as f:
+        f.write(f"t {g.n} {g.m // 2}\n")
+        for v in range(g.n):
+            f.write(f"v {v} {int(labels[v])} {int(deg[v])}\n")
+        for v in range(g.n):
+            for w in g.adj[g.offsets[v]:g.offsets[v + 1]]:
+                if v < w:
+                    f.write(f"e {v} {int(w)}\n")
+
+
+def test_load_labeled_fixture(gpm):
+    g, labels, L = gpm.load_labeled(os.path.join(GOLDEN, "tiny_labeled.graph"))
+    assert g.offsets.tolist() == [0, 2, 5, 8, 11, 12]
+    assert g.adj.tolist() == [1, 2, 0, 2, 3, 0, 1, 3, 1, 2, 4, 3]
+    assert labels.tolist() == [1, 0, 2, 1, 0] and L == 3
+
+
+def test_load_labeled_roundtrip_hprd(gpm, tmp_path):
+    from test_ceci import load_golden
+    g, labels, _, _, _ = load_golden(gpm)
+    p = str(tmp_path / "hprd.graph")
+    _write_tve(p, g, labels)
+    h, lab, L = gpm.load_labeled(p)
+    assert np.array_equal(h.offsets, g.offsets) and np.array_equal(h.adj, g.adj)
+    assert np.array_equal(lab, labels) and L == int(labels.max()) + 1
+
+
+@pytest.mark.parametrize("text", [
+    "x 2 1\nv 0 0 1\nv 1 0 1\ne 0 1\n",          # bad header
+    "t 2 1\nv 1 0 1\nv 0 0 1\ne 0 1\n",          # vertices out of order
+    "t 2 1\nv 0 0 1\nv 1 0 2\ne 0 1\n",          # declared degree not met
+    "t 2 1\nv 0 0 1\nv 1 0 1\ne 0 1\ne 0 1\n",   # more edges than declared
+    "t 2 1\nv 0 0 1\nv 1 0 1\ne 0 0\n",          # self loop
+    "t 3 1\nv 0 0 1\nv 1 0 1\ne 0 1\n",          # fewer vertices than header
+])
+def test_load_labeled_rejects(gpm, tmp_path, text):
+    p = tmp_path / "bad.graph"
+    p.write_text(text)
+    with pytest.raises(gpm.GpmError):
+        gpm.load_labeled(str(p))
+    with pytest.raises(gpm.GpmError):
+        gpm.load_labeled(str(tmp_path / "missing.graph"))
