@@ -5,12 +5,13 @@
 // reference's CPU CECI matcher prints as "#Embeddings"
 // (sub_ceci/myceci.cpp:540-668).  The GPU path is new work; its pieces:
 //
-//  * QueryingOrder (host).  Root = the query vertex minimising
-//    |V(label)| / degree (the CECI start rule, myceci.cpp:15-31).  The plain
-//    order is the BFS order from the root (myceci.cpp:65-103); the selective
-//    order instead adds, at each step, the vertex with the most already-
-//    placed neighbours (ties: rarer label), so that candidate sets are
-//    intersections of several lists as early as possible.
+//  * QueryingOrder (host).  Plain policy: root = the query vertex minimising
+//    |V(label)| / degree (the CECI start rule, myceci.cpp:15-31) and the BFS
+//    order from it (myceci.cpp:65-103).  Learned policy: a cost-based order
+//    from the data graph's label statistics -- for every start vertex the
+//    order grows by the vertex that keeps the expected number of partial
+//    embeddings smallest, and the start with the least expected total work
+//    wins.  The same estimate sizes the autotuner's chunks.
 //  * Candidate filter at every depth: right label, enough degree and, per
 //    neighbour label, enough neighbours of that label (the NLF filter of
 //    myceci.cpp:120-169), read in O(1) per label from the label-partitioned
@@ -250,6 +251,59 @@ struct HostQuery {
   std::vector<uint32_t> label;
 };
 
+// Label statistics of the data graph used by the cost estimates: |V(l)| and
+// E(a, b) = adjacency entries from label a to label b.
+struct LabelStats {
+  const gpm_graph* g;
+  double nv(uint32_t l) const {
+    return std::max(1.0, (double)(g->h_lvoff[l + 1] - g->h_lvoff[l]));
+  }
+  double epair(uint32_t a, uint32_t b) const {
+    if (!g->h_lpair.empty()) return g->h_lpair[(size_t)a * g->n_labels + b];
+    return (double)g->m * nv(a) * nv(b) / ((double)g->n * (double)g->n);
+  }
+  // Expected label-lb-slice length of a vertex of label la.
+  double slice(uint32_t la, uint32_t lb) const { return epair(la, lb) / nv(la); }
+  // Probability that a given label-lb vertex is adjacent to a label-la one.
+  double adj_prob(uint32_t la, uint32_t lb) const {
+    return std::min(1.0, epair(la, lb) / (nv(la) * nv(lb)));
+  }
+};
+
+// One DFS depth under the estimate: the anchor (shortest expected slice) is
+// streamed, every entry is probed against the other backward neighbours and
+// survives each with the adjacency probability.  Updates nodes (expected DFS
+// nodes at this depth per root) and returns the entries streamed+probed.
+double estimate_depth(const LabelStats& st, const uint32_t* label_of_depth, uint32_t bn,
+                      uint32_t label, double* nodes) {
+  double slice = 1e300;
+  uint32_t anchor = 0;
+  for (uint32_t m = bn; m; m &= m - 1) {
+    const uint32_t b = (uint32_t)__builtin_ctz(m);
+    const double len = st.slice(label_of_depth[b], label);
+    if (len < slice) {
+      slice = len;
+      anchor = b;
+    }
+  }
+  if (slice >= 1e300) slice = 0.0;
+  double pass = 1.0;
+  for (uint32_t m = bn & ~(1u << anchor); m; m &= m - 1)
+    pass *= st.adj_prob(label_of_depth[(uint32_t)__builtin_ctz(m)], label);
+  const double work = *nodes * slice * (double)__builtin_popcount(bn);
+  *nodes *= slice * pass;
+  return work;
+}
+
+// Expected work (label-slice entries streamed, times the membership probes
+// each entry takes) of the DFS below one root of plan p.
+double estimate_root_cost(const gpm_graph* g, const QPlan& p) {
+  const LabelStats st{g};
+  double nodes = 1.0, work = 0.0;
+  for (uint32_t d = 1; d < p.n; ++d) work += estimate_depth(st, p.label, p.bn[d], p.label[d], &nodes);
+  return std::max(1.0, work);
+}
+
 // Builds the plan of one query; returns false when it cannot match at all
 // (a label the data graph does not have).
 bool make_plan(const gpm_graph* g, const HostQuery& q, int policy, QPlan* p) {
@@ -270,27 +324,55 @@ bool make_plan(const gpm_graph* g, const HostQuery& q, int policy, QPlan* p) {
   std::vector<uint32_t> order;
   std::vector<char> placed(n, 0);
   if (policy == GPM_SCHED_LEARNED) {
-    order.push_back(start);
-    placed[start] = 1;
-    while (order.size() < n) {
-      int bu = -1, bconn = -1;
-      double brare = 1e300;
-      for (uint32_t u = 0; u < n; ++u) {
-        if (placed[u]) continue;
-        int conn = 0;
-        for (uint32_t w : q.adj[u]) conn += placed[w];
-        if (conn == 0) continue;
-        const double rare = lab_count(q.label[u]) / std::max<double>(1.0, q.adj[u].size());
-        if (conn > bconn || (conn == bconn && rare < brare)) {
-          bconn = conn;
-          brare = rare;
-          bu = (int)u;
+    // Cost-based order: from every start vertex, grow the order greedily by
+    // the vertex that keeps the expected number of partial embeddings
+    // smallest; keep the start whose whole order has the least expected work
+    // (root candidates x expected work per root).
+    const LabelStats st{g};
+    double best_total = 1e300;
+    for (uint32_t s0 = 0; s0 < n; ++s0) {
+      std::vector<uint32_t> ord{s0};
+      std::vector<char> in(n, 0);
+      in[s0] = 1;
+      uint32_t lab_of[kMaxQ];
+      std::vector<uint32_t> at(n, 0);
+      lab_of[0] = q.label[s0];
+      double nodes = 1.0, work = 1.0;
+      while (ord.size() < n) {
+        int bu = -1;
+        double bnodes = 0.0, bwork = 0.0;
+        int bconn = -1;
+        for (uint32_t u = 0; u < n; ++u) {
+          if (in[u]) continue;
+          uint32_t bn = 0;
+          for (uint32_t w : q.adj[u])
+            if (in[w]) bn |= 1u << at[w];
+          if (!bn) continue;
+          double nd = nodes;
+          const double wk = estimate_depth(st, lab_of, bn, q.label[u], &nd);
+          const int conn = __builtin_popcount(bn);
+          if (bu < 0 || nd < bnodes || (nd == bnodes && conn > bconn)) {
+            bu = (int)u;
+            bnodes = nd;
+            bwork = wk;
+            bconn = conn;
+          }
         }
+        if (bu < 0) return false;  // disconnected (checked by the caller)
+        at[bu] = (uint32_t)ord.size();
+        lab_of[ord.size()] = q.label[bu];
+        in[bu] = 1;
+        ord.push_back((uint32_t)bu);
+        nodes = bnodes;
+        work += bwork;
       }
-      if (bu < 0) return false;  // disconnected (checked by the caller)
-      placed[bu] = 1;
-      order.push_back((uint32_t)bu);
+      const double total = st.nv(q.label[s0]) * work;
+      if (total < best_total) {
+        best_total = total;
+        order = ord;
+      }
     }
+    for (uint32_t u : order) placed[u] = 1;
   } else {
     std::queue<uint32_t> bq;
     bq.push(start);
@@ -447,45 +529,6 @@ int build_graph(MatchBatch* b) {
   GPM_HIP(hipGraphAddKernelNode(&mn, b->graph, &zn, 1, &mp));
   GPM_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   return GPM_OK;
-}
-
-// Expected work (label-slice entries streamed, times the membership probes
-// each entry takes) of the DFS below one root of plan p, from the data
-// graph's label statistics: at depth d the streamed slice is the anchor's
-// label-d slice (expected length E(l_b, l_d) / |V(l_b)| for a backward
-// neighbour b, the shortest taken), and an entry survives each further
-// backward neighbour b' with probability E(l_b', l_d) / (|V(l_b')| |V(l_d)|).
-double estimate_root_cost(const gpm_graph* g, const QPlan& p) {
-  const uint32_t L = g->n_labels;
-  auto nv = [&](uint32_t l) {
-    return std::max(1.0, (double)(g->h_lvoff[l + 1] - g->h_lvoff[l]));
-  };
-  auto epair = [&](uint32_t a, uint32_t b) {
-    if (!g->h_lpair.empty()) return g->h_lpair[(size_t)a * L + b];
-    return (double)g->m * nv(a) * nv(b) / ((double)g->n * (double)g->n);
-  };
-  double nodes = 1.0, work = 0.0;
-  for (uint32_t d = 1; d < p.n; ++d) {
-    double slice = 1e300;
-    uint32_t anchor = 0;
-    for (uint32_t bn = p.bn[d]; bn; bn &= bn - 1) {
-      const uint32_t b = (uint32_t)__builtin_ctz(bn);
-      const double len = epair(p.label[b], p.label[d]) / nv(p.label[b]);
-      if (len < slice) {
-        slice = len;
-        anchor = b;
-      }
-    }
-    if (slice >= 1e300) slice = 0.0;
-    double pass = 1.0;
-    for (uint32_t bn = p.bn[d] & ~(1u << anchor); bn; bn &= bn - 1) {
-      const uint32_t b = (uint32_t)__builtin_ctz(bn);
-      pass *= std::min(1.0, epair(p.label[b], p.label[d]) / (nv(p.label[b]) * nv(p.label[d])));
-    }
-    work += nodes * slice * (double)__builtin_popcount(p.bn[d]);
-    nodes *= slice * pass;
-  }
-  return std::max(1.0, work);
 }
 
 // Plans, filters roots, tunes and lays out a batch.

@@ -71,3 +71,16 @@ def adjacency_matrix(g):
     for u in range(g.n):
         A[u, g.adj[g.offsets[u]:g.offsets[u + 1]]] = 1
     return A
+
+
+def write_labeled(path, g, labels):
+    """Writes g (symmetric CSR) in the sub_ceci text format ("t", "v", "e")."""
+    deg = g.degrees()
+    with open(path, "w") as f:
+        f.write(f"t {g.n} {g.m // 2}\n")
+        for v in range(g.n):
+            f.write(f"v {v} {int(labels[v])} {int(deg[v])}\n")
+        for v in range(g.n):
+            for w in g.adj[g.offsets[v]:g.offsets[v + 1]]:
+                if v < w:
+                    f.write(f"e {v} {int(w)}\n")

@@ -98,3 +98,26 @@ def test_unknown_label_and_errors(gpm):
     with gpm.DeviceGraph(g, oriented=False) as dg:
         with pytest.raises(gpm.GpmError):  # unlabeled data graph
             dg.match_batch([(2, np.array([[0, 1]], dtype=np.uint32), np.zeros(2, dtype=np.uint32))])
+
+
+def test_match_driver_reference_format(gpm, tmp_path):
+    """The CECI-style driver reads the reference's text format and prints
+    the reference's "#Embeddings" lines."""
+    import os
+    import subprocess
+    g, labels, qs, expected, _ = load_golden(gpm)
+    data = str(tmp_path / "data.graph")
+    graphs.write_labeled(data, g, labels)
+    args = [os.path.join(os.path.dirname(gpm.LIB_PATH), "..", "bin", "test_match"), "-d", data]
+    picks = (0, 57, 199)
+    for i in picks:
+        qn, e, ql = qs[i]
+        qg = gpm.csr_from_edges(qn, e[:, 0], e[:, 1], gpm.CSR_CLEAN)
+        p = str(tmp_path / f"q{i}.graph")
+        graphs.write_labeled(p, qg, ql)
+        args += ["-q", p]
+    for order in ("plain", "learned"):
+        out = subprocess.run(args + ["-order", order], capture_output=True, text=True,
+                             timeout=120, check=True).stdout
+        got = [int(line.split(":")[1]) for line in out.splitlines() if line.startswith("#Embeddings")]
+        assert got == [int(expected[i]) for i in picks]

@@ -106,18 +106,6 @@ def test_rmat_skew(gpm):
 
 # ---------------------------------------------------------------- sub_ceci text format
 
-def _write_tve(path, g, labels):
-    deg = g.degrees()
-    with open(path, "w") as f:
-        f.write(f"t {g.n} {g.m // 2}\n")
-        for v in range(g.n):
-            f.write(f"v {v} {int(labels[v])} {int(deg[v])}\n")
-        for v in range(g.n):
-            for w in g.adj[g.offsets[v]:g.offsets[v + 1]]:
-                if v < w:
-                    f.write(f"e {v} {int(w)}\n")
-
-
 def test_load_labeled_fixture(gpm):
     g, labels, L = gpm.load_labeled(os.path.join(GOLDEN, "tiny_labeled.graph"))
     assert g.offsets.tolist() == [0, 2, 5, 8, 11, 12]
@@ -129,7 +117,7 @@ def test_load_labeled_roundtrip_hprd(gpm, tmp_path):
     from test_ceci import load_golden
     g, labels, _, _, _ = load_golden(gpm)
     p = str(tmp_path / "hprd.graph")
-    _write_tve(p, g, labels)
+    graphs.write_labeled(p, g, labels)
     h, lab, L = gpm.load_labeled(p)
     assert np.array_equal(h.offsets, g.offsets) and np.array_equal(h.adj, g.adj)
     assert np.array_equal(lab, labels) and L == int(labels.max()) + 1
