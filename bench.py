@@ -171,6 +171,20 @@ def extra_match(scale, n_batch):
     return out
 
 
+def copy_calibration(mib):
+    """A plain copy of known size: src read once, dst written once."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,8 +196,14 @@ def main():
     ap.add_argument("--extras", default="tc,sched",
                     help="comma list of extra workloads: tc, sched, match (or none)")
     ap.add_argument("--sched-scale", type=int, default=22)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (to scale "
+                         "profiler byte counters against a known traffic) and exit")
     ap.add_argument("--match-scale", type=int, default=18)
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the "
               "multi-GPU split of roots is described in DESIGN.md but not built.",

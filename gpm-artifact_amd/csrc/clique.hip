@@ -235,8 +235,7 @@ __device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t* s_wsum
 // MAXW > 0 (rows in LDS, W <= MAXW): rows are read MAXW words at a time
 // with no per-word branch, so all loads of a row are in flight together and
 // one wait covers them (words past W belong to the next row or to the tail
-// of the block's LDS, and are masked out by m[x] = 0 for x >= W).  For K=5
-// the rows l of row_i & row_j are taken two at a time for the same reason.
+// of the block's LDS, and are masked out by m[x] = 0 for x >= W).
 // MAXW == 0 is the generic path for rows in HBM.
 template <int K, int MAXW, int BS>
 __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, uint32_t d,
@@ -297,27 +296,14 @@ __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, 
           for (int x = 0; x < MAXW; ++x) {
             uint64_t mm = m[x];
             while (mm) {
-              const uint32_t l1 = x * 64 + ctz64(mm);
+              const uint32_t l = x * 64 + ctz64(mm);
               mm &= mm - 1;
-              const unsigned long long* r1 = rows + (size_t)l1 * W;
-              if (MAXW <= 8 && mm) {
-                const uint32_t l2 = x * 64 + ctz64(mm);
-                mm &= mm - 1;
-                const unsigned long long* r2 = rows + (size_t)l2 * W;
-                uint64_t a[MAXW > 0 ? MAXW : 1], b[MAXW > 0 ? MAXW : 1];
+              const unsigned long long* rl = rows + (size_t)l * W;
+              uint64_t a[MAXW > 0 ? MAXW : 1];
 #pragma unroll
-                for (int y = 0; y < MAXW; ++y) a[y] = r1[y];
+              for (int y = 0; y < MAXW; ++y) a[y] = rl[y];
 #pragma unroll
-                for (int y = 0; y < MAXW; ++y) b[y] = r2[y];
-#pragma unroll
-                for (int y = 0; y < MAXW; ++y) c += __popcll(m[y] & a[y]) + __popcll(m[y] & b[y]);
-              } else {
-                uint64_t a[MAXW > 0 ? MAXW : 1];
-#pragma unroll
-                for (int y = 0; y < MAXW; ++y) a[y] = r1[y];
-#pragma unroll
-                for (int y = 0; y < MAXW; ++y) c += __popcll(m[y] & a[y]);
-              }
+              for (int y = 0; y < MAXW; ++y) c += __popcll(m[y] & a[y]);
             }
           }
         }

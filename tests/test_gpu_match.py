@@ -68,8 +68,10 @@ def test_random_vs_oracle(gpm, orc, seed, n_labels):
 
 def test_sampled_queries_vs_oracle(gpm, orc):
     import workloads
-    g, labels = workloads.labeled_graph(12, seed=5)
-    qs = workloads.sampled_queries(g, labels, 24, seed=9, max_size=8)
+    # Scale 11, queries of 4-6 vertices: ~5.7e8 embeddings, which the CPU
+    # oracle enumerates one by one in a few seconds.
+    g, labels = workloads.labeled_graph(11, seed=5)
+    qs = workloads.sampled_queries(g, labels, 24, seed=9, max_size=6)
     want = orc.ceci_batch(g.offsets, g.adj, labels, qs)
     assert np.all(want >= 1)  # cut out of the data graph
     with gpm.DeviceGraph(g, oriented=False, labels=labels, n_labels=8) as dg:
