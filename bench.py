@@ -31,6 +31,21 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "TC/k-clique edges/sec, matches/sec; learned-vs-plain sched speedup, exact"
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "clique_k4k5_rmat20_hbm_traffic.md")
+
+
+def measured_traffic(kernel, scale):
+    """HBM bytes per launch of `kernel` from the committed PMC summary (FETCH_SIZE +
+    WRITE_SIZE, copy-calibrated), for the scale-20 workload it was taken on."""
+    if scale != 20 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    with open(TRAFFIC_PROFILE) as f:
+        for line in f:
+            if line.startswith("traffic_json:"):
+                return json.loads(line.split(":", 1)[1]).get(kernel)
+    return None
+
+
 def tier_of(deg, k, caps):
     wave, lds, wide = caps
     t = np.full(len(deg), -1, dtype=np.int64)
@@ -282,7 +297,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dom, args.scale),
             "algorithmic_bytes_per_launch": alg_bytes,
             "mean_launch_ms": dom_ms,
         },
