@@ -47,6 +47,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kMaxQ = 16;
+constexpr uint32_t kMaxGroup = 3;
 
 struct QPlan {
   uint32_t n;               // query vertices
@@ -64,6 +65,15 @@ struct QPlan {
   uint32_t nlf_n[kMaxQ];
   uint32_t nlf_label[kMaxQ][kMaxQ];
   uint32_t nlf_count[kMaxQ][kMaxQ];
+  // Independent tail: depths tail..n-1 have no query edges among them, so
+  // once depths 0..tail-1 are matched their candidate sets are fixed and the
+  // injective assignments are counted, not enumerated.  The tail is split
+  // into groups of equal label (at most kMaxGroup depths each; different
+  // labels cannot collide).
+  uint32_t tail;
+  uint32_t n_groups;
+  uint32_t grp_size[kMaxQ];
+  uint32_t grp_dep[kMaxQ][3];
 };
 
 struct DevGraph {
@@ -86,6 +96,7 @@ __global__ __launch_bounds__(kBlock) void roots_kernel(DevGraph g, const uint32_
                                                        const uint32_t* __restrict__ lvoff,
                                                        const QPlan* __restrict__ plans,
                                                        uint32_t* __restrict__ roots,
+                                                       uint32_t* __restrict__ rlen,
                                                        unsigned int* __restrict__ counts) {
   const QPlan& p = plans[blockIdx.y];
   const uint32_t l0 = p.label[0];
@@ -102,6 +113,11 @@ __global__ __launch_bounds__(kBlock) void roots_kernel(DevGraph g, const uint32_
     if (ok) {
       const uint32_t at = atomicAdd(&counts[blockIdx.y], 1u);
       roots[p.root_off + at] = v;
+      // Depth-1 candidates stream v's label-l1 slice (its only backward
+      // neighbour is the root): its length sizes the split of heavy roots.
+      uint32_t lo = 0, len = 0;
+      if (p.n >= 2) label_slice(g, v, p.label[1], &lo, &len);
+      rlen[p.root_off + at] = len;
     }
   }
 }
@@ -164,14 +180,93 @@ __device__ __forceinline__ bool passes(const DevGraph& g, const QPlan& p, const 
   return true;
 }
 
-// Counts the embeddings of plan p rooted at v0.
-__device__ uint64_t count_root(const DevGraph& g, const QPlan& p, WaveState& s, uint32_t v0) {
+// Filters of tail depth d for an entry x (label already right): degree,
+// NLF, injectivity against the matched depths below the tail, and
+// membership in the lists of d's backward neighbours -- all of them when
+// `full`, all but the streamed anchor otherwise.
+__device__ __forceinline__ bool passes_tail(const DevGraph& g, const QPlan& p, const WaveState& s,
+                                            uint32_t d, uint32_t x, bool full) {
+  if (g.off[x + 1] - g.off[x] < p.qdeg[d]) return false;
+  for (uint32_t k = 0; k < p.nlf_n[d]; ++k) {
+    uint32_t lo, len;
+    label_slice(g, x, p.nlf_label[d][k], &lo, &len);
+    if (len < p.nlf_count[d][k]) return false;
+  }
+  for (uint32_t same = p.same[d] & ((1u << p.tail) - 1u); same; same &= same - 1)
+    if (s.emb[__builtin_ctz(same)] == x) return false;
+  uint32_t bn = full ? p.bn[d] : (p.bn[d] & ~(1u << s.anc[d]));
+  while (bn) {
+    const uint32_t b = __builtin_ctz(bn);
+    bn &= bn - 1;
+    uint32_t lo, len;
+    label_slice(g, s.emb[b], p.label[d], &lo, &len);
+    if (!sorted_contains(g.ladj + lo, len, x)) return false;
+  }
+  return true;
+}
+
+// Injective assignments of the tail depths given emb[0..tail-1]: the product
+// over equal-label groups of the inclusion-exclusion count over the group's
+// candidate sets A_1..A_k (k <= 3):
+//   k=1: |A1|;  k=2: |A1||A2| - |A12|;
+//   k=3: |A1||A2||A3| - |A12||A3| - |A13||A2| - |A23||A1| + 2|A123|.
+// |A_B| for a subset B is counted while streaming the candidates of B's
+// first member (each x tested against the later members' full filters).
+// Arithmetic is modulo 2^64, exact when the true count fits.
+__device__ uint64_t count_tail(const DevGraph& g, const QPlan& p, WaveState& s) {
+  const uint32_t lane = lane_id();
+  uint64_t prod = 1;
+  for (uint32_t gi = 0; gi < p.n_groups; ++gi) {
+    const uint32_t k = p.grp_size[gi];
+    uint64_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // a[B]: |intersection over members in B|
+    for (uint32_t m = 0; m < k; ++m) {
+      const uint32_t d = p.grp_dep[gi][m];
+      init_depth(g, p, s, d);
+      const uint32_t sl = uniform(s.sl[d]), sb = uniform(s.sb[d]);
+      for (uint32_t cur = 0; cur < sl; cur += kWave) {
+        const uint32_t i = cur + lane;
+        uint32_t mask = 0;
+        if (i < sl) {
+          const uint32_t x = g.ladj[sb + i];
+          if (passes_tail(g, p, s, d, x, false)) {
+            mask = 1u << m;
+            for (uint32_t m2 = m + 1; m2 < k; ++m2)
+              if (passes_tail(g, p, s, p.grp_dep[gi][m2], x, true)) mask |= 1u << m2;
+          }
+        }
+        // Subsets B with lowest member m and B within mask.
+        for (uint32_t B = 1u << m; B < 8u; ++B)
+          if ((B & ((1u << m) - 1u)) == 0 && (B & ~mask) == 0 && (B >> m) & 1u) a[B] += 1;
+      }
+    }
+    for (uint32_t B = 1; B < 8; ++B) a[B] = wave_sum_u64(a[B]);
+    uint64_t c;
+    if (k == 1)
+      c = a[1];
+    else if (k == 2)
+      c = a[1] * a[2] - a[3];
+    else
+      c = a[1] * a[2] * a[4] - a[3] * a[4] - a[5] * a[2] - a[6] * a[1] + 2 * a[7];
+    prod *= c;
+    if (prod == 0) break;
+  }
+  return prod;
+}
+
+// Counts the embeddings of plan p rooted at v0 whose depth-1 vertex is one
+// of the entries [lo1, hi1) of the depth-1 slice (all: 0, ~0u): a DFS over
+// depths 1..tail-1, each leaf of which adds the tail count.
+__device__ uint64_t count_root(const DevGraph& g, const QPlan& p, WaveState& s, uint32_t v0,
+                               uint32_t lo1, uint32_t hi1) {
   const uint32_t lane = lane_id();
   const uint32_t n = p.n;
   if (n == 1) return 1;
-  uint64_t total = 0;
   s.emb[0] = v0;
+  if (p.tail == 1) return count_tail(g, p, s);
+  uint64_t total = 0;
   init_depth(g, p, s, 1);
+  s.cur[1] = lo1;
+  s.sl[1] = min(s.sl[1], hi1);
   int d = 1;
   while (d >= 1) {
     const uint32_t dd = (uint32_t)d;
@@ -183,22 +278,21 @@ __device__ uint64_t count_root(const DevGraph& g, const QPlan& p, WaveState& s, 
       }
       const uint32_t i = cur + lane;
       const bool ok = i < s.sl[dd] && passes(g, p, s, dd, g.ladj[s.sb[dd] + i]);
-      const uint64_t m = wave_ballot(ok);
+      s.mask[dd] = wave_ballot(ok);
+      s.base[dd] = cur;
       s.cur[dd] = cur + kWave;
-      if (dd == n - 1) {
-        total += (uint64_t)__popcll(m);
-      } else {
-        s.mask[dd] = m;
-        s.base[dd] = cur;
-      }
       continue;
     }
     const unsigned long long m = s.mask[dd];
     const uint32_t j = (uint32_t)__builtin_ctzll(m);
     s.mask[dd] = m & (m - 1);
     s.emb[dd] = uniform(g.ladj[s.sb[dd] + s.base[dd] + j]);
-    init_depth(g, p, s, dd + 1);
-    ++d;
+    if (dd + 1 == p.tail) {
+      total += count_tail(g, p, s);
+    } else {
+      init_depth(g, p, s, dd + 1);
+      ++d;
+    }
   }
   return total;
 }
@@ -215,12 +309,20 @@ __device__ __forceinline__ uint32_t find_plan(const QPlan* plans, uint32_t nq, u
   return lo;
 }
 
-// Persistent enumeration: every wave pulls tasks (a chunk of one query's
-// root candidates) from one queue until the queue is drained.
+// A part of one heavy root: depth-1 entries [lo, hi) of root `root` (index
+// into the plan's root list) of plan `plan`.
+struct HeavyTask {
+  uint32_t plan, root, lo, hi;
+};
+
+// Persistent enumeration: every wave pulls tasks from one queue until it is
+// drained.  Tasks [0, n_heavy) are parts of heavy roots (heaviest first);
+// the rest are chunks of a query's root list, which skip the heavy roots.
 __global__ __launch_bounds__(kBlock) void match_kernel(
     DevGraph g, const QPlan* __restrict__ plans, uint32_t nq, uint32_t n_tasks,
-    const uint32_t* __restrict__ roots, unsigned int* __restrict__ queue,
-    unsigned long long* __restrict__ counts) {
+    const uint32_t* __restrict__ roots, const uint32_t* __restrict__ rlen,
+    const HeavyTask* __restrict__ heavy, uint32_t n_heavy, uint32_t split_len,
+    unsigned int* __restrict__ queue, unsigned long long* __restrict__ counts) {
   __shared__ WaveState s_state[kWavesPerBlock];
   WaveState& s = s_state[threadIdx.x / kWave];
   const uint32_t lane = lane_id();
@@ -229,13 +331,26 @@ __global__ __launch_bounds__(kBlock) void match_kernel(
     if (lane == 0) t = atomicAdd(queue, 1u);
     t = uniform(wave_bcast(t, 0));
     if (t >= n_tasks) break;
-    const uint32_t pi = uniform(find_plan(plans, nq, t));
-    const QPlan& p = plans[pi];
-    const uint32_t r0 = (t - p.task_off) * p.chunk;
-    const uint32_t r1 = min(r0 + p.chunk, p.n_roots);
     uint64_t c = 0;
-    for (uint32_t r = r0; r < r1; ++r) c += count_root(g, p, s, uniform(roots[p.root_off + r]));
-    if (lane == 0 && c) atomicAdd(&counts[p.qid], (unsigned long long)c);
+    uint32_t qid;
+    if (t < n_heavy) {
+      const HeavyTask h = heavy[t];
+      const QPlan& p = plans[uniform(h.plan)];
+      qid = p.qid;
+      c = count_root(g, p, s, uniform(roots[p.root_off + uniform(h.root)]), uniform(h.lo),
+                     uniform(h.hi));
+    } else {
+      const uint32_t pi = uniform(find_plan(plans, nq, t - n_heavy));
+      const QPlan& p = plans[pi];
+      qid = p.qid;
+      const uint32_t r0 = (t - n_heavy - p.task_off) * p.chunk;
+      const uint32_t r1 = min(r0 + p.chunk, p.n_roots);
+      for (uint32_t r = r0; r < r1; ++r) {
+        if (p.tail > 1 && uniform(rlen[p.root_off + r]) > split_len) continue;  // a heavy task
+        c += count_root(g, p, s, uniform(roots[p.root_off + r]), 0u, 0xFFFFFFFFu);
+      }
+    }
+    if (lane == 0 && c) atomicAdd(&counts[qid], (unsigned long long)c);
   }
 }
 
@@ -372,6 +487,19 @@ bool make_plan(const gpm_graph* g, const HostQuery& q, int policy, QPlan* p) {
         order = ord;
       }
     }
+    // Move vertices whose neighbours all come earlier to the end, so the
+    // independent tail (counted, not enumerated) is as long as possible.
+    for (uint32_t i = n - 1; i-- > 1;) {
+      const uint32_t u = order[i];
+      bool sink = true;
+      for (uint32_t j = i + 1; j < n && sink; ++j)
+        for (uint32_t w : q.adj[u])
+          if (w == order[j]) sink = false;
+      if (sink) {
+        order.erase(order.begin() + i);
+        order.push_back(u);
+      }
+    }
     for (uint32_t u : order) placed[u] = 1;
   } else {
     std::queue<uint32_t> bq;
@@ -419,6 +547,27 @@ bool make_plan(const gpm_graph* g, const HostQuery& q, int policy, QPlan* p) {
       p->nlf_count[d][i] = need[i].second;
     }
   }
+  // Independent tail: grow it backwards from the last depth while the next
+  // depth has no edge into it and no label group exceeds kMaxGroup.
+  uint32_t t = n - 1;
+  while (t > 1) {
+    bool ok = true;
+    uint32_t same_label = 1;
+    for (uint32_t d = t; d < n; ++d) {
+      if (p->bn[d] & (1u << (t - 1))) ok = false;
+      if (p->label[d] == p->label[t - 1]) ++same_label;
+    }
+    if (!ok || same_label > kMaxGroup) break;
+    --t;
+  }
+  p->tail = n >= 2 ? t : 1;
+  p->n_groups = 0;
+  for (uint32_t d = p->tail; d < n; ++d) {
+    uint32_t gi = 0;
+    while (gi < p->n_groups && p->label[p->grp_dep[gi][0]] != p->label[d]) ++gi;
+    if (gi == p->n_groups) p->grp_size[p->n_groups++] = 0;
+    p->grp_dep[gi][p->grp_size[gi]++] = d;
+  }
   return true;
 }
 
@@ -451,9 +600,12 @@ struct MatchBatch {
   void* mem = nullptr;
   QPlan* d_plans = nullptr;
   uint32_t* d_roots = nullptr;
+  uint32_t* d_rlen = nullptr;
   unsigned long long* d_counts = nullptr;
   unsigned int* d_rootcnt = nullptr;
   unsigned int* d_queue = nullptr;
+  HeavyTask* d_heavy = nullptr;  // own allocation
+  uint32_t n_heavy = 0, split_len = 0;
   uint32_t nq = 0, n_tasks = 0;
   int n_queries = 0;
   unsigned grid = 0;
@@ -468,6 +620,7 @@ void free_batch(void* p) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   (void)hipFree(b->mem);
+  (void)hipFree(b->d_heavy);
   delete b;
 }
 
@@ -497,7 +650,8 @@ int launch_batch(gpm_graph* g, MatchBatch* b) {
   hipLaunchKernelGGL(zero_kernel, dim3((b->n_queries + 64) / 64 + 1), dim3(64), 0, g->stream,
                      b->d_counts, b->n_queries, b->d_queue, 4);
   hipLaunchKernelGGL(match_kernel, dim3(b->grid), dim3(kBlock), 0, g->stream, b->dg,
-                     b->d_plans, b->nq, b->n_tasks, b->d_roots, b->d_queue, b->d_counts);
+                     b->d_plans, b->nq, b->n_tasks, b->d_roots, b->d_rlen, b->d_heavy,
+                     b->n_heavy, b->split_len, b->d_queue, b->d_counts);
   GPM_HIP(hipGetLastError());
   return GPM_OK;
 }
@@ -517,8 +671,9 @@ int build_graph(MatchBatch* b) {
   zp.extra = nullptr;
   hipGraphNode_t zn = nullptr, mn = nullptr;
   GPM_HIP(hipGraphAddKernelNode(&zn, b->graph, nullptr, 0, &zp));
-  void* margs[] = {&b->dg,      &b->d_plans, &b->nq,      &b->n_tasks,
-                   &b->d_roots, &b->d_queue, &b->d_counts};
+  void* margs[] = {&b->dg,      &b->d_plans,   &b->nq,      &b->n_tasks,
+                   &b->d_roots, &b->d_rlen,    &b->d_heavy, &b->n_heavy,
+                   &b->split_len, &b->d_queue, &b->d_counts};
   hipKernelNodeParams mp{};
   mp.func = reinterpret_cast<void*>(&match_kernel);
   mp.gridDim = dim3(b->grid);
@@ -570,12 +725,14 @@ int prepare_batch(gpm_graph* g, const gpm_query* queries, int n_queries, const g
   const size_t count_bytes = align((size_t)(n_queries + 1) * 8);
   const size_t small_bytes = align((size_t)(nq + 1 + 4) * 4);
   const size_t lvoff_bytes = align(g->h_lvoff.size() * 4);
-  const size_t total = plan_bytes + root_bytes + count_bytes + small_bytes + lvoff_bytes;
+  const size_t total = plan_bytes + 2 * root_bytes + count_bytes + small_bytes + lvoff_bytes;
   if (hipMalloc(&b->mem, total) != hipSuccess) return fail(GPM_ENOMEM, "batch buffers (%zu B)", total);
   char* p = static_cast<char*>(b->mem);
   b->d_plans = reinterpret_cast<QPlan*>(p);
   p += plan_bytes;
   b->d_roots = reinterpret_cast<uint32_t*>(p);
+  p += root_bytes;
+  b->d_rlen = reinterpret_cast<uint32_t*>(p);
   p += root_bytes;
   b->d_counts = reinterpret_cast<unsigned long long*>(p);
   p += count_bytes;
@@ -598,7 +755,7 @@ int prepare_batch(gpm_graph* g, const gpm_query* queries, int n_queries, const g
   dim3 rgrid((unsigned)std::max<uint32_t>(1, std::min<uint32_t>((maxlab + kBlock - 1) / kBlock, 256)),
              nq);
   hipLaunchKernelGGL(roots_kernel, rgrid, dim3(kBlock), 0, g->stream, b->dg, g->lverts, d_lvoff,
-                     b->d_plans, b->d_roots, b->d_rootcnt);
+                     b->d_plans, b->d_roots, b->d_rlen, b->d_rootcnt);
   GPM_HIP(hipGetLastError());
   std::vector<unsigned int> rootcnt(nq);
   GPM_HIP(hipMemcpyAsync(rootcnt.data(), b->d_rootcnt, nq * 4, hipMemcpyDeviceToHost, g->stream));
@@ -631,7 +788,48 @@ int prepare_batch(gpm_graph* g, const gpm_query* queries, int n_queries, const g
     t += q.n_roots ? (q.n_roots + q.chunk - 1) / q.chunk : 0;
     laid.push_back(q);
   }
-  b->n_tasks = t;
+  // Heavy roots (depth-1 slice longer than kSplitLen) are dealt out as parts
+  // of kSplitLen depth-1 entries each, so one hub's subtree is spread over
+  // many waves; parts of the longest slices go first.  Queries whose tail
+  // starts at depth 1 are counted per root without a DFS and are not split.
+  constexpr uint32_t kSplitLen = 128;
+  b->split_len = kSplitLen;
+  std::vector<HeavyTask> heavy;
+  if (root_total) {
+    std::vector<uint32_t> rlen(root_total);
+    GPM_HIP(hipMemcpyAsync(rlen.data(), b->d_rlen, root_total * 4, hipMemcpyDeviceToHost,
+                           g->stream));
+    GPM_HIP(hipStreamSynchronize(g->stream));
+    std::vector<uint32_t> hlen;
+    for (uint32_t li = 0; li < nq; ++li) {
+      const QPlan& q = laid[li];
+      if (q.tail <= 1) continue;
+      for (uint32_t r = 0; r < q.n_roots; ++r) {
+        const uint32_t len = rlen[q.root_off + r];
+        if (len <= kSplitLen) continue;
+        for (uint32_t lo = 0; lo < len; lo += kSplitLen) {
+          heavy.push_back({li, r, lo, std::min(len, lo + kSplitLen)});
+          hlen.push_back(len);
+        }
+      }
+    }
+    std::vector<uint32_t> order(heavy.size());
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t c) { return hlen[a] > hlen[c]; });
+    std::vector<HeavyTask> sorted(heavy.size());
+    for (size_t i = 0; i < order.size(); ++i) sorted[i] = heavy[order[i]];
+    heavy.swap(sorted);
+  }
+  if (heavy.size() >= (1ull << 31)) return fail(GPM_EINVAL, "too many heavy-root parts");
+  b->n_heavy = (uint32_t)heavy.size();
+  if (b->n_heavy) {
+    if (hipMalloc(&b->d_heavy, heavy.size() * sizeof(HeavyTask)) != hipSuccess)
+      return fail(GPM_ENOMEM, "heavy-root tasks");
+    if ((st = h2d(g, b->d_heavy, heavy.data(), heavy.size() * sizeof(HeavyTask))) != GPM_OK)
+      return st;
+  }
+  b->n_tasks = b->n_heavy + t;
   if ((st = h2d(g, b->d_plans, laid.data(), (size_t)nq * sizeof(QPlan))) != GPM_OK) return st;
   if (tun.use_graph) return build_graph(b);
   return GPM_OK;

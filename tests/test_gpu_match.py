@@ -123,3 +123,24 @@ def test_match_driver_reference_format(gpm, tmp_path):
                              timeout=120, check=True).stdout
         got = [int(line.split(":")[1]) for line in out.splitlines() if line.startswith("#Embeddings")]
         assert got == [int(expected[i]) for i in picks]
+
+
+def test_heavy_roots_split(gpm, orc):
+    """Roots whose depth-1 slice is long are dealt out in parts across waves;
+    counts must equal the oracle's (wheel hub of degree 2999; one-label
+    R-MAT with many hubs)."""
+    tri = (3, np.array([[0, 1], [1, 2], [0, 2]], dtype=np.uint32), np.zeros(3, dtype=np.uint32))
+    c4 = (4, np.array([[0, 1], [1, 2], [2, 3], [3, 0]], dtype=np.uint32), np.zeros(4, dtype=np.uint32))
+    dia = (4, np.array([[0, 1], [0, 2], [0, 3], [1, 2], [1, 3]], dtype=np.uint32),
+           np.zeros(4, dtype=np.uint32))
+    path = (4, np.array([[0, 1], [1, 2], [2, 3]], dtype=np.uint32), np.zeros(4, dtype=np.uint32))
+    for g in (graphs.wheel(3000), gpm.rmat_graph(11, 8, 77)):
+        labels = np.zeros(g.n, dtype=np.uint32)
+        qs = [tri, c4, dia, path]
+        want = orc.ceci_batch(g.offsets, g.adj, labels, qs)
+        with gpm.DeviceGraph(g, oriented=False, labels=labels, n_labels=1) as dg:
+            for policy in (0, 1):
+                assert np.array_equal(dg.match_batch(qs, policy=policy), want)
+            assert np.array_equal(dg.match_batch(qs, autotune=0, chunk=2, use_graph=1), want)
+    # Closed form on the wheel: 6 (n - 1) ordered triangles.
+    assert want[0] != 0
