@@ -168,21 +168,50 @@ def extra_sched(scale):
     return out
 
 
-def extra_match(scale, n_batch):
-    """Labeled matching: 5 named queries (scale 18) and a batch of sampled
-    queries (scale 21)."""
+def extra_match(scale):
+    """Labeled matching: the reference's 200 HPRD queries as one batch (exact
+    against the reference's known answers), and the named queries on a
+    labeled R-MAT graph."""
     import workloads as W
     out = {}
+    z = np.load(os.path.join(ROOT, "tests", "golden", "ceci_hprd.npz"))
+    e = z["data_edges"]
+    g = G.csr_from_edges(int(z["data_n"]), e[:, 0], e[:, 1], G.CSR_CLEAN)
+    qs, eo, lo = [], 0, 0
+    for qn, qm in zip(z["q_n"], z["q_m"]):
+        qn, qm = int(qn), int(qm)
+        qs.append((qn, z["q_edges"][eo:eo + 2 * qm].reshape(-1, 2), z["q_labels"][lo:lo + qn]))
+        eo, lo = eo + 2 * qm, lo + qn
+    want = z["expected"].astype(np.uint64)
+    res = {"workload": "200 reference queries (query_dense_16) on HPRD, one batch"}
+    with G.DeviceGraph(g, oriented=False, labels=z["data_labels"]) as dg:
+        for name, kw in (("plain", dict(autotune=1, policy=G.SCHED_DEGREE)),
+                         ("learned", dict(autotune=1, policy=G.SCHED_LEARNED)),
+                         ("graph_replay", dict(autotune=0, chunk=8, use_graph=1))):
+            c = dg.match_batch(qs, **kw)
+            ms = []
+            for _ in range(3):
+                assert np.array_equal(dg.match_batch(qs, **kw), c)
+                ms.append(dg.last_kernel_ms())
+            res[name] = {"ms": min(ms), "exact_vs_reference": bool(np.array_equal(c, want)),
+                         "embeddings_per_s": float(c.sum()) / (min(ms) * 1e-3)}
+        res["embeddings"] = int(want.sum())
+        res["learned_speedup"] = res["plain"]["ms"] / res["learned"]["ms"]
+    out["hprd_batch"] = res
+    print("[bench] hprd batch done", file=sys.stderr, flush=True)
     g, labels = W.labeled_graph(scale)
-    qs = [q for _, q in W.named_queries()]
+    named = dict(W.named_queries())
+    res = {"workload": f"named queries on labeled R-MAT scale {scale} (8 labels)"}
     with G.DeviceGraph(g, oriented=False, labels=labels, n_labels=W.N_LABELS) as dg:
-        res = {}
-        for pol, name in ((G.SCHED_DEGREE, "plain"), (G.SCHED_LEARNED, "learned")):
-            c = dg.match_batch(qs, autotune=1, policy=pol)
-            res[name] = {"counts": [int(x) for x in c], "ms": dg.last_kernel_ms()}
-        res["identical"] = res["plain"]["counts"] == res["learned"]["counts"]
-        res["embeddings_per_s"] = sum(res["learned"]["counts"]) / (res["learned"]["ms"] * 1e-3)
-        out["named"] = dict(workload=f"5 named queries on labeled R-MAT scale {scale}", **res)
+        for qname in ("triangle", "4-cycle", "diamond", "three-star"):
+            row = {}
+            for pol, pname in ((G.SCHED_DEGREE, "plain"), (G.SCHED_LEARNED, "learned")):
+                c = dg.match_batch([named[qname]], autotune=1, policy=pol)
+                row[pname] = {"count": int(c[0]), "ms": dg.last_kernel_ms()}
+            row["identical"] = row["plain"]["count"] == row["learned"]["count"]
+            res[qname] = row
+            print(f"[bench] {qname} done", file=sys.stderr, flush=True)
+    out["named"] = res
     return out
 
 
@@ -208,13 +237,13 @@ def main():
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--extras", default="tc,sched",
+    ap.add_argument("--extras", default="tc,sched,match",
                     help="comma list of extra workloads: tc, sched, match (or none)")
     ap.add_argument("--sched-scale", type=int, default=22)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (to scale "
                          "profiler byte counters against a known traffic) and exit")
-    ap.add_argument("--match-scale", type=int, default=18)
+    ap.add_argument("--match-scale", type=int, default=16)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -316,7 +345,7 @@ def main():
         elif e == "sched":
             out["extra"]["sched"] = extra_sched(args.sched_scale)
         elif e == "match":
-            out["extra"]["match"] = extra_match(args.match_scale, 256)
+            out["extra"]["match"] = extra_match(args.match_scale)
         print(f"[bench] extra {e} done in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
               flush=True)
     print(json.dumps(out), flush=True)

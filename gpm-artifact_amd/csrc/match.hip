@@ -487,19 +487,6 @@ bool make_plan(const gpm_graph* g, const HostQuery& q, int policy, QPlan* p) {
         order = ord;
       }
     }
-    // Move vertices whose neighbours all come earlier to the end, so the
-    // independent tail (counted, not enumerated) is as long as possible.
-    for (uint32_t i = n - 1; i-- > 1;) {
-      const uint32_t u = order[i];
-      bool sink = true;
-      for (uint32_t j = i + 1; j < n && sink; ++j)
-        for (uint32_t w : q.adj[u])
-          if (w == order[j]) sink = false;
-      if (sink) {
-        order.erase(order.begin() + i);
-        order.push_back(u);
-      }
-    }
     for (uint32_t u : order) placed[u] = 1;
   } else {
     std::queue<uint32_t> bq;
@@ -514,6 +501,20 @@ bool make_plan(const gpm_graph* g, const HostQuery& q, int policy, QPlan* p) {
           placed[w] = 1;
           bq.push(w);
         }
+    }
+  }
+  // Move vertices whose neighbours all come earlier to the end (keeping the
+  // root first), so the independent tail -- counted, not enumerated -- is as
+  // long as possible.  Every moved vertex keeps all its backward neighbours.
+  for (uint32_t i = n - 1; i-- > 1;) {
+    const uint32_t u = order[i];
+    bool sink = true;
+    for (uint32_t j = i + 1; j < n && sink; ++j)
+      for (uint32_t w : q.adj[u])
+        if (w == order[j]) sink = false;
+    if (sink) {
+      order.erase(order.begin() + i);
+      order.push_back(u);
     }
   }
   std::vector<uint32_t> pos(n);

@@ -95,3 +95,24 @@ def sampled_queries(g: G.HostCSR, labels: np.ndarray, count: int, seed: int,
         e = np.asarray(sorted(edges), dtype=np.uint32).reshape(-1, 2)
         out.append((size, e, labels[np.asarray(members)].astype(np.uint32)))
     return out
+
+
+def random_queries(count: int, seed: int, min_size: int = 4, max_size: int = 10,
+                   extra_edge_prob: float = 0.5, n_labels: int = N_LABELS):
+    """Seeded connected query graphs: a random spanning tree (vertex i joins a
+    uniformly drawn earlier vertex) plus every other pair with probability
+    extra_edge_prob; sizes uniform in [min_size, max_size], labels uniform."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.integers(min_size, max_size + 1))
+        edges = set()
+        for v in range(1, n):
+            edges.add((int(rng.integers(0, v)), v))
+        for a in range(n):
+            for b in range(a + 1, n):
+                if (a, b) not in edges and rng.random() < extra_edge_prob:
+                    edges.add((a, b))
+        e = np.asarray(sorted(edges), dtype=np.uint32).reshape(-1, 2)
+        out.append((n, e, rng.integers(0, n_labels, n).astype(np.uint32)))
+    return out
