@@ -44,6 +44,8 @@ constexpr uint32_t kBlockCap = 512;
 constexpr int kLdsBlock = 512;
 constexpr int kWideBlock = 1024;
 constexpr int kHbmBlock = 1024;
+// Consecutive list entries one thread streams in the row build.
+constexpr uint32_t kSeg = 4;
 enum { kCount = 0, kBytes = 1, kWork = 2 };
 
 __device__ __forceinline__ bool use_probe(uint32_t len, uint32_t d) {
@@ -461,19 +463,31 @@ __global__ __launch_bounds__(BS) void clique_block_kernel(
       s_pre[threadIdx.x] = incl;
       __syncthreads();
       const uint32_t total = uniform(s_pre[BS - 1]);
-      for (uint32_t f = threadIdx.x; f < total; f += BS) {
-        const uint32_t j = upper_index(s_pre, BS, f);
-        const uint32_t excl = j ? s_pre[j - 1] : 0u;
-        const uint32_t row_i = c0 + j;
-        const uint32_t x = adj[begs[row_i] + (f - excl)];
-        int idx;
-        if (LDS_ROWS) {
-          idx = lds_hash_find(keys, vals, hmask, x);
-        } else {
-          const uint32_t p = upper_index(S, d, x);
-          idx = (p > 0 && S[p - 1] == x) ? (int)(p - 1) : -1;
+      // Each thread takes kSeg consecutive entries: one owner search for the
+      // first, then the owner row advances along the prefix sums.
+      for (uint32_t f0 = threadIdx.x * kSeg; f0 < total; f0 += BS * kSeg) {
+        uint32_t j = upper_index(s_pre, BS, f0);
+        uint32_t excl = j ? s_pre[j - 1] : 0u;
+        uint32_t end = s_pre[j];
+#pragma unroll
+        for (uint32_t k = 0; k < kSeg; ++k) {
+          const uint32_t f = f0 + k;
+          if (f >= total) break;
+          while (f >= end) {  // next non-empty row; ends since f < total
+            excl = end;
+            end = s_pre[++j];
+          }
+          const uint32_t row_i = c0 + j;
+          const uint32_t x = adj[begs[row_i] + (f - excl)];
+          int idx;
+          if (LDS_ROWS) {
+            idx = lds_hash_find(keys, vals, hmask, x);
+          } else {
+            const uint32_t p = upper_index(S, d, x);
+            idx = (p > 0 && S[p - 1] == x) ? (int)(p - 1) : -1;
+          }
+          if (idx >= 0) atomicOr(&rows[(size_t)row_i * W + (idx >> 6)], 1ull << (idx & 63));
         }
-        if (idx >= 0) atomicOr(&rows[(size_t)row_i * W + (idx >> 6)], 1ull << (idx & 63));
       }
       __syncthreads();
     }

@@ -90,6 +90,15 @@ int get_tiers(gpm_graph* g, const TierRequest& req, const RootTiers** out) {
   }
   std::vector<uint32_t> block_parts, block2_parts, big_parts;
   uint32_t n_split = 0;
+  if (!learned) {
+    // Plain degree scheduler: each tier in decreasing out-degree, so the
+    // largest roots start first (the reference's degree-sorted order).
+    auto by_deg = [&](uint32_t a, uint32_t b) {
+      const uint32_t da = g->h_off[a + 1] - g->h_off[a], db = g->h_off[b + 1] - g->h_off[b];
+      return da > db || (da == db && a < b);
+    };
+    for (auto* tier : {&wave, &block, &block2, &big}) std::stable_sort(tier->begin(), tier->end(), by_deg);
+  }
   if (learned) {
     // Heaviest predicted roots first (longest-processing-time order), so the
     // expensive subtrees start early and the tail of the launch is short.

@@ -1,8 +1,9 @@
 // Root tiers: the host-side split of root vertices into size classes that
 // decides which kernel serves each root (one wave, one block with LDS rows,
 // one block with rows in HBM).  The plain degree scheduler orders each tier
-// by vertex id; the learned scheduler orders it by predicted cost (heaviest
-// first) and may move a root to a wider tier when its predicted cost says so.
+// by decreasing out-degree; the learned scheduler orders it by predicted cost
+// (heaviest first), may move a root to a wider tier when its predicted cost
+// says so, and splits predicted-heavy roots into parts.
 #pragma once
 
 #include <cstdint>
