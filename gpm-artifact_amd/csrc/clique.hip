@@ -16,6 +16,12 @@
 // lengths), or, for lists much longer than S, by binary-searching each S[j]
 // in N+(S[i]) and taking the 64-bit ballot as the row word.
 //
+// The kernels read the DAG relabelled in a topological order (gpm_graph's
+// toff/tadj, built once per handle), so S = N+(v0) is sorted topologically
+// and every row is upper triangular: row i only holds positions > i.  For
+// k=5 the popcount of row_i & row_j & row_l then starts at l's word, which
+// skips about half to two thirds of the words.  Counts do not depend on ids.
+//
 // Three tiers by |S| (tiers.hpp): one wave per root with rows in registers
 // (|S| <= 64, one 64-bit word per row, lane i owns row i); one 256-thread
 // block per root with rows in LDS (|S| <= 512); one block per root with rows
@@ -300,12 +306,14 @@ __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, 
             while (mm) {
               const uint32_t l = x * 64 + ctz64(mm);
               mm &= mm - 1;
+              // Row l holds only positions > l (rows are upper triangular),
+              // so its words below x are zero and are not read.
               const unsigned long long* rl = rows + (size_t)l * W;
               uint64_t a[MAXW > 0 ? MAXW : 1];
 #pragma unroll
-              for (int y = 0; y < MAXW; ++y) a[y] = rl[y];
+              for (int y = x; y < MAXW; ++y) a[y] = rl[y];
 #pragma unroll
-              for (int y = 0; y < MAXW; ++y) c += __popcll(m[y] & a[y]);
+              for (int y = x; y < MAXW; ++y) c += __popcll(m[y] & a[y]);
             }
           }
         }
@@ -328,7 +336,7 @@ __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, 
               const uint32_t l = x * 64 + ctz64(mm);
               mm &= mm - 1;
               const unsigned long long* rl = rows + (size_t)l * W;
-              for (uint32_t y = 0; y < W; ++y) c += __popcll(ri[y] & rj[y] & rl[y]);
+              for (uint32_t y = x; y < W; ++y) c += __popcll(ri[y] & rj[y] & rl[y]);
             }
           }
         }
@@ -605,7 +613,7 @@ int launch_lds_tier(gpm_graph* g, const uint32_t* roots, const uint32_t* parts,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const unsigned blocks =
       (unsigned)std::min<uint64_t>(n_roots, (uint64_t)cus * lds_blocks_per_cu(cap, BS));
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(BS), lds, g->stream, g->off, g->adj, roots,
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(BS), lds, g->stream, g->toff, g->tadj, roots,
                      parts, n_roots, cap, table, queue, nullptr, nullptr, out, work);
   GPM_HIP(hipGetLastError());
   if (MODE == kCount) timing_mark(g, name);
@@ -631,7 +639,7 @@ int launch_tiers(gpm_graph* g, const RootTiers* t, unsigned long long* out,
     uint32_t* gmeta = reinterpret_cast<uint32_t*>(static_cast<char*>(g->scratch) + row_bytes);
     hipLaunchKernelGGL((clique_block_kernel<K, MODE, false, 0, kHbmBlock>), dim3(blocks),
                        dim3(kHbmBlock), 0,
-                       g->stream, g->off, g->adj, t->d_big, t->d_big_parts, t->n_big, cap, 0u,
+                       g->stream, g->toff, g->tadj, t->d_big, t->d_big_parts, t->n_big, cap, 0u,
                        q + 2, grows,
                        gmeta, out, work);
     GPM_HIP(hipGetLastError());
@@ -654,7 +662,7 @@ int launch_tiers(gpm_graph* g, const RootTiers* t, unsigned long long* out,
     const uint64_t waves = std::min<uint64_t>(waves_wanted, (uint64_t)cus * 32);
     const unsigned blocks = (unsigned)std::max<uint64_t>(1, (waves + kWavesPerBlock - 1) / kWavesPerBlock);
     hipLaunchKernelGGL((clique_wave_kernel<K, MODE>), dim3(blocks), dim3(kBlock), 0, g->stream,
-                       g->off, g->adj, t->d_wave, t->n_wave, chunk, q, out, work);
+                       g->toff, g->tadj, t->d_wave, t->n_wave, chunk, q, out, work);
     GPM_HIP(hipGetLastError());
     if (MODE == kCount) timing_mark(g, "clique_wave");
   }
@@ -709,7 +717,10 @@ int gpm_clique(gpm_graph* g, int k, const gpm_sched* sched, uint64_t* out) {
                 "unknown scheduling policy");
   GPM_CHECK_ARG(!sched || sched->policy != GPM_SCHED_LEARNED || sched->root_cost,
                 "LEARNED policy needs root_cost");
+  int st = ensure_topo(g);
+  if (st != GPM_OK) return st;
   TierRequest req = request_from(sched, (uint32_t)(k - 1));
+  req.rank = g->h_rank.data();
   if (req.policy == GPM_SCHED_LEARNED) {
     req.heavy_cost = learned_heavy_cost(req.cost, g->n);
     const uint32_t cus = (uint32_t)device_cus(g);
@@ -718,8 +729,7 @@ int gpm_clique(gpm_graph* g, int k, const gpm_sched* sched, uint64_t* out) {
     req.resident_big = cus;
   }
   const RootTiers* t = nullptr;
-  int st = get_tiers(g, req, &t);
-  if (st != GPM_OK) return st;
+  if ((st = get_tiers(g, req, &t)) != GPM_OK) return st;
   if ((st = timing_begin(g)) != GPM_OK) return st;
   if ((st = zero_counters(g, 1)) != GPM_OK) return st;
   if ((st = launch_k<kCount>(g, k, t, g->counters, nullptr)) != GPM_OK) return st;
@@ -746,9 +756,12 @@ int gpm_clique_plan_bytes(gpm_graph* g, int k, uint64_t* bytes) {
   GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "needs an ORIENTED graph");
   GPM_CHECK_ARG(k >= 3 && k <= 5, "k must be 3, 4 or 5 (got %d)", k);
   if (!g->plan_bytes[k]) {
+    int st = ensure_topo(g);
+    if (st != GPM_OK) return st;
     TierRequest req = request_from(nullptr, 1);
+    req.rank = g->h_rank.data();
     const RootTiers* t = nullptr;
-    int st = get_tiers(g, req, &t);
+    st = get_tiers(g, req, &t);
     if (st != GPM_OK) return st;
     if ((st = zero_counters(g, 1)) != GPM_OK) return st;
     if ((st = launch_k<kBytes>(g, k, t, g->counters, nullptr)) != GPM_OK) return st;
@@ -766,21 +779,28 @@ int gpm_clique_root_work(gpm_graph* g, int k, uint64_t* work) {
   GPM_CHECK_ARG(g && work, "null pointer");
   GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "needs an ORIENTED graph");
   GPM_CHECK_ARG(k >= 3 && k <= 5, "k must be 3, 4 or 5 (got %d)", k);
+  int st = ensure_topo(g);
+  if (st != GPM_OK) return st;
   TierRequest req = request_from(nullptr, 1);
+  req.rank = g->h_rank.data();
   const RootTiers* t = nullptr;
-  int st = get_tiers(g, req, &t);
+  st = get_tiers(g, req, &t);
   if (st != GPM_OK) return st;
   unsigned long long* d_work = nullptr;
   GPM_HIP(hipMalloc(&d_work, (size_t)std::max(g->n, 1u) * 8));
   GPM_HIP(hipMemsetAsync(d_work, 0, (size_t)std::max(g->n, 1u) * 8, g->stream));
   st = launch_k<kWork>(g, k, t, g->counters, d_work);
+  // The kernels log by topological id; report by vertex id.
+  std::vector<uint64_t> by_rank(g->n);
   if (st == GPM_OK) {
-    if (hipMemcpyAsync(work, d_work, (size_t)g->n * 8, hipMemcpyDeviceToHost, g->stream) !=
-            hipSuccess ||
+    if (hipMemcpyAsync(by_rank.data(), d_work, (size_t)g->n * 8, hipMemcpyDeviceToHost,
+                       g->stream) != hipSuccess ||
         hipStreamSynchronize(g->stream) != hipSuccess)
       st = fail(GPM_EHIP, "work log copy failed");
   }
   hipFree(d_work);
+  if (st == GPM_OK)
+    for (uint32_t v = 0; v < g->n; ++v) work[v] = by_rank[g->h_rank[v]];
   return st;
 }
 

@@ -51,6 +51,15 @@ void build_label_csr(uint32_t n, uint32_t L, const uint32_t* offsets,
                      const uint32_t* adj, const uint32_t* labels,
                      std::vector<uint32_t>& ladj, std::vector<uint32_t>& loff);
 
+// Topological relabel of a DAG CSR: rank[v] = position of v in a
+// topological order (Kahn's algorithm, ready vertices taken in id order);
+// new_off/new_adj is the same DAG over the new ids with sorted lists, so
+// every edge goes from a lower to a higher id.  Returns false when the graph
+// has a cycle (then the outputs are unspecified).
+bool topo_relabel(uint32_t n, const uint32_t* offsets, const uint32_t* adj,
+                  std::vector<uint32_t>& rank, std::vector<uint32_t>& new_off,
+                  std::vector<uint32_t>& new_adj);
+
 template <typename T>
 T* xmalloc(size_t count) {
   size_t bytes = count * sizeof(T);

@@ -102,6 +102,36 @@ int h2d(gpm_graph* g, void* dst, const void* src, size_t bytes) {
   return GPM_OK;
 }
 
+int ensure_topo(gpm_graph* g) {
+  if (g->toff) return GPM_OK;
+  std::vector<uint32_t> h_adj(std::max<uint64_t>(g->m, 1));
+  if (g->m) {
+    GPM_HIP(hipMemcpyAsync(h_adj.data(), g->adj, g->m * 4, hipMemcpyDeviceToHost, g->stream));
+    GPM_HIP(hipStreamSynchronize(g->stream));
+  }
+  std::vector<uint32_t> rank, noff, nadj;
+  if (!topo_relabel(g->n, g->h_off.data(), h_adj.data(), rank, noff, nadj))
+    return fail(GPM_EINVAL, "the ORIENTED graph has a cycle: k-clique counting needs an acyclic "
+                            "orientation");
+  uint32_t *toff = nullptr, *tadj = nullptr;
+  if (hipMalloc(&toff, noff.size() * 4) != hipSuccess ||
+      hipMalloc(&tadj, std::max<size_t>(nadj.size(), 1) * 4) != hipSuccess) {
+    (void)hipFree(toff);
+    return fail(GPM_ENOMEM, "topological copy of the DAG");
+  }
+  int st = h2d(g, toff, noff.data(), noff.size() * 4);
+  if (st == GPM_OK) st = h2d(g, tadj, nadj.data(), nadj.size() * 4);
+  if (st != GPM_OK) {
+    (void)hipFree(toff);
+    (void)hipFree(tadj);
+    return st;
+  }
+  g->toff = toff;
+  g->tadj = tadj;
+  g->h_rank.swap(rank);
+  return GPM_OK;
+}
+
 int ensure_scratch(gpm_graph* g, size_t bytes) {
   if (g->scratch_bytes >= bytes) return GPM_OK;
   if (g->scratch) {
@@ -182,6 +212,8 @@ void release(gpm_graph* g) {
   hipFree(g->ladj);
   hipFree(g->loff);
   hipFree(g->lverts);
+  (void)hipFree(g->toff);
+  (void)hipFree(g->tadj);
   hipFree(g->scratch);
   hipFree(g->counters);
   for (auto& e : g->mark_ev)

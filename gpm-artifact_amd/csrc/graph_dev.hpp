@@ -58,6 +58,12 @@ struct gpm_graph {
   gpm_graph* dag = nullptr;
   // Last prepared matching batch (match.hip), replayed on identical calls.
   void* match_cache = nullptr;
+  // ORIENTED graphs: the same DAG relabelled in a topological order
+  // (toff/tadj, every edge from a lower to a higher id; h_rank[v] = new id
+  // of v), built on first use by the k-clique kernels.
+  uint32_t* toff = nullptr;
+  uint32_t* tadj = nullptr;
+  std::vector<uint32_t> h_rank;
 };
 
 namespace gpm {
@@ -82,6 +88,8 @@ int timing_mark(gpm_graph* g, const char* name);
 int timing_end(gpm_graph* g);
 // Number of compute units of the handle's device.
 int device_cus(const gpm_graph* g);
+// Builds g->toff/tadj/h_rank if missing; GPM_EINVAL if the graph has a cycle.
+int ensure_topo(gpm_graph* g);
 // Releases the cached matching batch of g (match.hip).
 void free_match_cache(gpm_graph* g);
 

@@ -300,6 +300,46 @@ int gpm_load_mtx(const char* path, uint32_t* n, uint32_t** src, uint32_t** dst,
   return GPM_OK;
 }
 
+}  // extern "C"
+
+namespace gpm {
+
+bool topo_relabel(uint32_t n, const uint32_t* offsets, const uint32_t* adj,
+                  std::vector<uint32_t>& rank, std::vector<uint32_t>& new_off,
+                  std::vector<uint32_t>& new_adj) {
+  const uint64_t m = offsets[n];
+  std::vector<uint32_t> indeg(n, 0);
+  for (uint64_t e = 0; e < m; ++e) indeg[adj[e]]++;
+  std::vector<uint32_t> order;
+  order.reserve(n);
+  for (uint32_t v = 0; v < n; ++v)
+    if (!indeg[v]) order.push_back(v);
+  for (size_t h = 0; h < order.size(); ++h) {
+    const uint32_t v = order[h];
+    for (uint32_t e = offsets[v]; e < offsets[v + 1]; ++e)
+      if (--indeg[adj[e]] == 0) order.push_back(adj[e]);
+  }
+  if (order.size() != n) return false;
+  rank.assign(n, 0);
+  for (uint32_t r = 0; r < n; ++r) rank[order[r]] = r;
+  new_off.assign((size_t)n + 1, 0);
+  for (uint32_t r = 0; r < n; ++r) new_off[r + 1] = new_off[r] + (offsets[order[r] + 1] - offsets[order[r]]);
+  new_adj.resize(m);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t r = 0; r < (int64_t)n; ++r) {
+    const uint32_t v = order[r];
+    uint32_t* out = new_adj.data() + new_off[r];
+    const uint32_t len = offsets[v + 1] - offsets[v];
+    for (uint32_t i = 0; i < len; ++i) out[i] = rank[adj[offsets[v] + i]];
+    std::sort(out, out + len);
+  }
+  return true;
+}
+
+}  // namespace gpm
+
+extern "C" {
+
 int gpm_load_labeled(const char* path, uint32_t* n, uint32_t** offsets, uint32_t** adj,
                      uint64_t* m, uint32_t** labels, uint32_t* n_labels) {
   clear_error();

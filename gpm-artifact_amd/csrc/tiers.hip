@@ -23,10 +23,10 @@ std::string key_of(const TierRequest& r, uint32_t n) {
   if (r.n_root_list)
     cs ^= checksum(reinterpret_cast<const float*>(r.root_list), r.n_root_list) * 31 +
           r.n_root_list;
-  std::snprintf(buf, sizeof(buf), "%u/%u/%u/%u/%u/%u/%d/%u/%g/%u/%u/%u/%llx", r.min_deg,
+  std::snprintf(buf, sizeof(buf), "%u/%u/%u/%u/%u/%u/%d/%u/%g/%u/%u/%u/%d/%llx", r.min_deg,
                 r.wave_cap, r.block_cap, r.block2_cap, r.root_begin, r.root_end, r.policy, r.chunk,
                 r.heavy_cost, r.resident_block, r.resident_block2, r.resident_big,
-                (unsigned long long)cs);
+                r.rank != nullptr, (unsigned long long)cs);
   return buf;
 }
 
@@ -158,6 +158,9 @@ int get_tiers(gpm_graph* g, const TierRequest& req, const RootTiers** out) {
   t.max_block2_deg = maxb2;
   t.max_big_deg = maxg;
   t.wave_chunk = req.chunk ? req.chunk : 8;
+  if (req.rank)
+    for (auto* tier : {&wave, &block, &block2, &big})
+      for (uint32_t& v : *tier) v = req.rank[v];
   int st;
   if ((st = upload(g, wave, &t.d_wave)) != GPM_OK) return st;
   if ((st = upload(g, block, &t.d_block)) != GPM_OK) return st;

@@ -53,6 +53,9 @@ struct TierRequest {
   // root predicted to cost more than a quarter of a block's fair share of
   // its tier is split into parts (0 = never split).
   uint32_t resident_block = 0, resident_block2 = 0, resident_big = 0;
+  // When set, the uploaded root lists hold rank[v] instead of v (kernels
+  // that read the topologically relabelled DAG).
+  const uint32_t* rank = nullptr;
 };
 
 // Most parts a root is split into, and fewest rows a part is given.
