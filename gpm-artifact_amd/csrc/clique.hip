@@ -600,8 +600,8 @@ int lds_blocks_per_cu(uint32_t cap, int block_threads) {
 
 // One LDS-rows tier: rows of up to MAXW words, cap = its largest out-degree.
 template <int K, int MODE, int MAXW, int BS>
-int launch_lds_tier(gpm_graph* g, const uint32_t* roots, const uint32_t* parts,
-                    uint32_t n_roots, uint32_t cap,
+int launch_lds_tier(gpm_graph* g, hipStream_t stream, const uint32_t* roots,
+                    const uint32_t* parts, uint32_t n_roots, uint32_t cap,
                     unsigned int* queue, unsigned long long* out, unsigned long long* work,
                     const char* name) {
   const int cus = device_cus(g);
@@ -613,20 +613,48 @@ int launch_lds_tier(gpm_graph* g, const uint32_t* roots, const uint32_t* parts,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const unsigned blocks =
       (unsigned)std::min<uint64_t>(n_roots, (uint64_t)cus * lds_blocks_per_cu(cap, BS));
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(BS), lds, g->stream, g->toff, g->tadj, roots,
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(BS), lds, stream, g->toff, g->tadj, roots,
                      parts, n_roots, cap, table, queue, nullptr, nullptr, out, work);
   GPM_HIP(hipGetLastError());
-  if (MODE == kCount) timing_mark(g, name);
+  if (MODE == kCount && stream == g->stream) timing_mark(g, name);
   return GPM_OK;
 }
 
 template <int K, int MODE>
+int launch_wave_tier(gpm_graph* g, hipStream_t stream, const RootTiers* t, unsigned int* q,
+                     unsigned long long* out, unsigned long long* work) {
+  const int cus = device_cus(g);
+  const uint32_t chunk = t->wave_chunk;
+  const uint64_t waves_wanted = (t->n_wave + chunk - 1) / chunk;
+  const uint64_t waves = std::min<uint64_t>(waves_wanted, (uint64_t)cus * 32);
+  const unsigned blocks =
+      (unsigned)std::max<uint64_t>(1, (waves + kWavesPerBlock - 1) / kWavesPerBlock);
+  hipLaunchKernelGGL((clique_wave_kernel<K, MODE>), dim3(blocks), dim3(kBlock), 0, stream,
+                     g->toff, g->tadj, t->d_wave, t->n_wave, chunk, q, out, work);
+  GPM_HIP(hipGetLastError());
+  if (MODE == kCount && stream == g->stream) timing_mark(g, "clique_wave");
+  return GPM_OK;
+}
+
+// Launches every non-empty tier.  overlap == false: one stream, tiers one
+// after another.  overlap == true: the rows-in-HBM and wave tiers go to the
+// handle's second stream, so they run in the tails of (and, for the wave
+// tier, beside) the LDS tiers; per-kernel marks then cover the main stream.
+template <int K, int MODE>
 int launch_tiers(gpm_graph* g, const RootTiers* t, unsigned long long* out,
-                 unsigned long long* work) {
+                 unsigned long long* work, bool overlap) {
   const int cus = device_cus(g);
   unsigned int* q = t->queues;
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, g->stream, q, 4);
   GPM_HIP(hipGetLastError());
+  hipStream_t side = g->stream;
+  if (overlap) {
+    int st = ensure_aux(g);
+    if (st != GPM_OK) return st;
+    GPM_HIP(hipEventRecord(g->ev_fork, g->stream));
+    GPM_HIP(hipStreamWaitEvent(g->aux, g->ev_fork, 0));
+    side = g->aux;
+  }
   if (t->n_big) {
     const uint32_t cap = t->max_big_deg;
     const uint32_t W = (cap + 63) / 64;
@@ -638,44 +666,45 @@ int launch_tiers(gpm_graph* g, const RootTiers* t, unsigned long long* out,
     unsigned long long* grows = static_cast<unsigned long long*>(g->scratch);
     uint32_t* gmeta = reinterpret_cast<uint32_t*>(static_cast<char*>(g->scratch) + row_bytes);
     hipLaunchKernelGGL((clique_block_kernel<K, MODE, false, 0, kHbmBlock>), dim3(blocks),
-                       dim3(kHbmBlock), 0,
-                       g->stream, g->toff, g->tadj, t->d_big, t->d_big_parts, t->n_big, cap, 0u,
-                       q + 2, grows,
-                       gmeta, out, work);
+                       dim3(kHbmBlock), 0, side, g->toff, g->tadj, t->d_big, t->d_big_parts,
+                       t->n_big, cap, 0u, q + 2, grows, gmeta, out, work);
     GPM_HIP(hipGetLastError());
-    if (MODE == kCount) timing_mark(g, "clique_hbm_block");
+    if (MODE == kCount && !overlap) timing_mark(g, "clique_hbm_block");
   }
-  if (t->n_block) {
-    int st = launch_lds_tier<K, MODE, 8, kLdsBlock>(g, t->d_block, t->d_block_parts, t->n_block,
-                                         t->max_block_deg, q + 1, out, work, "clique_lds_block");
+  if (overlap && t->n_wave) {
+    int st = launch_wave_tier<K, MODE>(g, side, t, q, out, work);
     if (st != GPM_OK) return st;
   }
   if (t->n_block2) {
-    int st = launch_lds_tier<K, MODE, 16, kWideBlock>(g, t->d_block2, t->d_block2_parts, t->n_block2,
-                                          t->max_block2_deg, q + 3, out, work,
-                                          "clique_lds_block_wide");
+    int st = launch_lds_tier<K, MODE, 16, kWideBlock>(g, g->stream, t->d_block2, t->d_block2_parts,
+                                                      t->n_block2, t->max_block2_deg, q + 3, out,
+                                                      work, "clique_lds_block_wide");
     if (st != GPM_OK) return st;
   }
-  if (t->n_wave) {
-    const uint32_t chunk = t->wave_chunk;
-    const uint64_t waves_wanted = (t->n_wave + chunk - 1) / chunk;
-    const uint64_t waves = std::min<uint64_t>(waves_wanted, (uint64_t)cus * 32);
-    const unsigned blocks = (unsigned)std::max<uint64_t>(1, (waves + kWavesPerBlock - 1) / kWavesPerBlock);
-    hipLaunchKernelGGL((clique_wave_kernel<K, MODE>), dim3(blocks), dim3(kBlock), 0, g->stream,
-                       g->toff, g->tadj, t->d_wave, t->n_wave, chunk, q, out, work);
-    GPM_HIP(hipGetLastError());
-    if (MODE == kCount) timing_mark(g, "clique_wave");
+  if (t->n_block) {
+    int st = launch_lds_tier<K, MODE, 8, kLdsBlock>(g, g->stream, t->d_block, t->d_block_parts,
+                                                    t->n_block, t->max_block_deg, q + 1, out, work,
+                                                    "clique_lds_block");
+    if (st != GPM_OK) return st;
+  }
+  if (!overlap && t->n_wave) {
+    int st = launch_wave_tier<K, MODE>(g, g->stream, t, q, out, work);
+    if (st != GPM_OK) return st;
+  }
+  if (overlap) {
+    GPM_HIP(hipEventRecord(g->ev_join, g->aux));
+    GPM_HIP(hipStreamWaitEvent(g->stream, g->ev_join, 0));
   }
   return GPM_OK;
 }
 
 template <int MODE>
 int launch_k(gpm_graph* g, int k, const RootTiers* t, unsigned long long* out,
-             unsigned long long* work) {
+             unsigned long long* work, bool overlap = false) {
   switch (k) {
-    case 3: return launch_tiers<3, MODE>(g, t, out, work);
-    case 4: return launch_tiers<4, MODE>(g, t, out, work);
-    default: return launch_tiers<5, MODE>(g, t, out, work);
+    case 3: return launch_tiers<3, MODE>(g, t, out, work, overlap);
+    case 4: return launch_tiers<4, MODE>(g, t, out, work, overlap);
+    default: return launch_tiers<5, MODE>(g, t, out, work, overlap);
   }
 }
 
@@ -732,7 +761,11 @@ int gpm_clique(gpm_graph* g, int k, const gpm_sched* sched, uint64_t* out) {
   if ((st = get_tiers(g, req, &t)) != GPM_OK) return st;
   if ((st = timing_begin(g)) != GPM_OK) return st;
   if ((st = zero_counters(g, 1)) != GPM_OK) return st;
-  if ((st = launch_k<kCount>(g, k, t, g->counters, nullptr)) != GPM_OK) return st;
+  // The learned scheduler also overlaps tiers: the HBM-row tier (split into
+  // parts) and the wave tier run on the second stream beside the LDS tiers.
+  // The plain scheduler launches the tiers one after another.
+  const bool overlap = req.policy == GPM_SCHED_LEARNED;
+  if ((st = launch_k<kCount>(g, k, t, g->counters, nullptr, overlap)) != GPM_OK) return st;
   if ((st = timing_end(g)) != GPM_OK) return st;
   unsigned long long h = 0;
   GPM_HIP(hipMemcpyAsync(&h, g->counters, sizeof(h), hipMemcpyDeviceToHost, g->stream));
