@@ -210,6 +210,8 @@ void release(gpm_graph* g) {
   if (!g) return;
   if (g->stream) hipStreamSynchronize(g->stream);
   free_match_cache(g);
+  if (g->tc_exec) (void)hipGraphExecDestroy(g->tc_exec);
+  if (g->tc_graph) (void)hipGraphDestroy(g->tc_graph);
   if (g->dag) gpm_graph_destroy(g->dag);
   hipFree(g->rank);
   free_tiers(g);

@@ -151,3 +151,20 @@ def test_error_paths(gpm):
     bad = gpm.HostCSR(np.array([0, 1], dtype=np.uint32), np.array([5], dtype=np.uint32))
     with pytest.raises(gpm.GpmError):
         gpm.DeviceGraph(bad, oriented=True)
+
+
+def test_tc_repeat_graph_replays(gpm, orc):
+    """Repeated counts (a hipGraph of independent counts for the edge variant)
+    equal the eager count on every replay, with eager work in between."""
+    g = gpm.er_graph(4096, 40000, 11)
+    d = gpm.orient(g)
+    want = orc.tc(d.offsets, d.adj)
+    with gpm.DeviceGraph(d, oriented=True) as dg:
+        for _ in range(3):
+            assert dg.tc_repeat(gpm.TC_EDGE, 64) == want
+            assert dg.tc(gpm.TC_EDGE) == want
+            assert dg.tc_repeat(gpm.TC_EDGE, 5) == want
+            assert dg.tc_repeat(gpm.TC_VERTEX, 3) == want
+            assert dg.clique(3) == want
+        with pytest.raises(gpm.GpmError):
+            dg.tc_repeat(gpm.TC_EDGE, 0)
