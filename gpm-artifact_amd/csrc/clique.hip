@@ -51,7 +51,7 @@ constexpr int kLdsBlock = 512;
 constexpr int kWideBlock = 1024;
 constexpr int kHbmBlock = 1024;
 // Consecutive list entries one thread streams in the row build.
-constexpr uint32_t kSeg = 4;
+constexpr uint32_t kSeg = 8;
 enum { kCount = 0, kBytes = 1, kWork = 2 };
 
 __device__ __forceinline__ bool use_probe(uint32_t len, uint32_t d) {
@@ -89,6 +89,7 @@ __global__ __launch_bounds__(kBlock) void clique_wave_kernel(
   __shared__ uint32_t s_pre[kWavesPerBlock][kWave];
   __shared__ uint32_t s_beg[kWavesPerBlock][kWave];
   __shared__ uint32_t s_len[kWavesPerBlock][kWave];
+  __shared__ uint32_t s_sv[kWavesPerBlock][kWave];
   const uint32_t lane = lane_id();
   const uint32_t w = threadIdx.x / kWave;
   uint32_t* keys = s_keys[w];
@@ -97,6 +98,7 @@ __global__ __launch_bounds__(kBlock) void clique_wave_kernel(
   uint32_t* pre = s_pre[w];
   uint32_t* beg = s_beg[w];
   uint32_t* lens = s_len[w];
+  uint32_t* svals = s_sv[w];
   constexpr uint32_t hmask = kWaveTable - 1;
   uint64_t acc = 0;
   while (true) {
@@ -137,14 +139,18 @@ __global__ __launch_bounds__(kBlock) void clique_wave_kernel(
           if (idx >= 0) atomicOr(&rows[j], 1ull << idx);
         }
       }
-      uint64_t pm = wave_ballot(probe);
-      while (pm) {
-        const uint32_t i = ctz64(pm);
-        pm &= pm - 1;
-        const uint32_t bi = wave_bcast(b, i), li = wave_bcast(len, i);
-        const bool found = act && sorted_contains(adj + bi, li, s);
-        const uint64_t m = wave_ballot(found);
-        if (lane == 0) rows[i] = m;
+      // Probed lists: the (probed row, element of S) pairs are dealt out
+      // over the lanes, so all binary searches of a root run side by side.
+      const uint64_t pm = wave_ballot(probe);
+      if (pm) {
+        svals[lane] = s;
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t np = (uint32_t)__popcll(pm);
+        for (uint32_t f = lane; f < np * d; f += kWave) {
+          const uint32_t k = f / d, t = f - k * d;
+          const uint32_t i = select_bit(pm, k);
+          if (sorted_contains(adj + beg[i], lens[i], svals[t])) atomicOr(&rows[i], 1ull << t);
+        }
       }
       __builtin_amdgcn_wave_barrier();
       const uint64_t row = act ? rows[lane] : 0ull;
@@ -405,6 +411,7 @@ __global__ __launch_bounds__(BS) void clique_block_kernel(
   __shared__ uint32_t s_pre[BS];
   __shared__ uint32_t s_wsum[(BS / kWave)];
   __shared__ uint32_t s_root;
+  __shared__ uint32_t s_np;
   __shared__ int s_any_probe;
   __shared__ unsigned long long s_red[(BS / kWave)];
   const uint32_t Wcap = (cap + 63) / 64;
@@ -502,17 +509,23 @@ __global__ __launch_bounds__(BS) void clique_block_kernel(
     // Probed lists: the ballot of "S[t] in N+(S[i])" over 64 consecutive t
     // is row word t/64.
     if (uniform((uint32_t)s_any_probe)) {
-      for (uint32_t i = 0; i < d; ++i) {
-        const uint32_t len = lens[i];
-        if (!use_probe(len, d)) continue;
-        const uint32_t* Ni = adj + begs[i];
-        for (uint32_t t0 = 0; t0 < d; t0 += BS) {
-          const uint32_t t = t0 + threadIdx.x;
-          const bool found = t < d && sorted_contains(Ni, len, S[t]);
-          const uint64_t m = wave_ballot(found);
-          const uint32_t word = t / 64;
-          if (lane_id() == 0 && word < W) rows[(size_t)i * W + word] = m;
+      // Probed rows of each BS-row chunk are listed in s_pre, then the
+      // (probed row, element of S) pairs are dealt out over the block.
+      for (uint32_t c0 = 0; c0 < d; c0 += BS) {
+        const uint32_t i = c0 + threadIdx.x;
+        const uint32_t flag = (i < d && use_probe(lens[i], d)) ? 1u : 0u;
+        const uint32_t incl = block_incl_scan(flag, s_wsum);
+        if (flag) s_pre[incl - 1] = i;
+        if (threadIdx.x == BS - 1) s_np = incl;
+        __syncthreads();
+        const uint32_t np = uniform(s_np);
+        for (uint32_t f = threadIdx.x; f < np * d; f += BS) {
+          const uint32_t k = f / d, t = f - k * d;
+          const uint32_t ri = s_pre[k];
+          if (sorted_contains(adj + begs[ri], lens[ri], S[t]))
+            atomicOr(&rows[(size_t)ri * W + (t >> 6)], 1ull << (t & 63));
         }
+        __syncthreads();
       }
     }
     __syncthreads();

@@ -168,3 +168,33 @@ def test_tc_repeat_graph_replays(gpm, orc):
             assert dg.clique(3) == want
         with pytest.raises(gpm.GpmError):
             dg.tc_repeat(gpm.TC_EDGE, 0)
+
+
+def test_probe_paths_closed_form(gpm):
+    """Roots whose out-neighbours have long lists (> max(512, 8 d)) build
+    their rows by binary search instead of streaming: small vertices (wave
+    tier) and medium vertices (LDS tier) hanging off a 700-clique.  With no
+    edges between non-clique vertices the count is closed-form:
+    C(700, k) + sum over non-clique v of C(t_v, k - 1), t_v = hubs of v."""
+    rng = np.random.default_rng(2024)
+    h, n_med, n_small = 700, 100, 2000
+    src, dst = [], []
+    for a in range(h):
+        for b in range(a + 1, h):
+            src.append(a)
+            dst.append(b)
+    ts = []
+    for v in range(h, h + n_med + n_small):
+        t = int(rng.integers(70, 90)) if v < h + n_med else int(rng.integers(4, 13))
+        for hub in rng.choice(h, size=t, replace=False):
+            src.append(v)
+            dst.append(int(hub))
+        ts.append(t)
+    g = gpm.csr_from_edges(h + n_med + n_small, src, dst, gpm.CSR_CLEAN)
+    with gpm.DeviceGraph(gpm.orient(g), oriented=True) as dg:
+        for k in (3, 4, 5):
+            want = comb(h, k) + sum(comb(t, k - 1) for t in ts)
+            assert dg.clique(k) == want
+            deg = np.diff(gpm.orient(g).offsets.astype(np.int64)).astype(np.float32)
+            assert dg.clique(k, policy=gpm.SCHED_LEARNED, cost=deg ** 2) == want
+        assert dg.tc(gpm.TC_EDGE) == comb(h, 3) + sum(comb(t, 2) for t in ts)
