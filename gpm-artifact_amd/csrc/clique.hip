@@ -54,8 +54,15 @@ constexpr int kHbmBlock = 1024;
 constexpr uint32_t kSeg = 8;
 enum { kCount = 0, kBytes = 1, kWork = 2 };
 
+// A list N+(S[i]) much longer than S is not streamed: each S[t] is looked up
+// in it by binary search instead (d searches of log2(len) steps against len
+// streamed entries).  Thresholds measured on R-MAT scale 20: lists longer
+// than max(64, 2d) for the wave tier and max(64, 4d) for the block tiers.
 __device__ __forceinline__ bool use_probe(uint32_t len, uint32_t d) {
-  return len > max(512u, 8u * d);
+  return len > max(64u, 4u * d);
+}
+__device__ __forceinline__ bool use_probe_wave(uint32_t len, uint32_t d) {
+  return len > max(64u, 2u * d);
 }
 
 __device__ __forceinline__ uint32_t ctz64(uint64_t x) { return (uint32_t)__builtin_ctzll(x); }
@@ -123,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void clique_wave_kernel(
         len = off[s + 1] - b;
       }
       lens[lane] = len;
-      const bool probe = act && use_probe(len, d);
+      const bool probe = act && use_probe_wave(len, d);
       const uint32_t incl = wave_incl_scan(probe ? 0u : len);
       pre[lane] = incl;
       beg[lane] = b;
