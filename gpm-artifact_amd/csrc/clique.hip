@@ -260,8 +260,9 @@ __device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t* s_wsum
 // MAXW == 0 is the generic path for rows in HBM.
 template <int K, int MAXW, int BS>
 __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, uint32_t d,
-                                                uint32_t W, uint32_t part, uint32_t n_parts,
-                                                uint32_t* s_pre, uint32_t* s_wsum) {
+                                                uint32_t W, uint32_t RS, uint32_t part,
+                                                uint32_t n_parts, uint32_t* s_pre,
+                                                uint32_t* s_wsum) {
   uint64_t c = 0;
   // This task counts rows part, part + n_parts, ...: nr of them.
   const uint32_t nr = d > part ? (d - part + n_parts - 1) / n_parts : 0u;
@@ -269,7 +270,7 @@ __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, 
     const uint32_t t0 = c0 + threadIdx.x;
     uint32_t pc = 0;
     if (t0 < nr) {
-      const unsigned long long* r0 = rows + (size_t)(part + t0 * n_parts) * W;
+      const unsigned long long* r0 = rows + (size_t)(part + t0 * n_parts) * RS;
       if (MAXW > 0) {
 #pragma unroll
         for (int w = 0; w < MAXW; ++w) {
@@ -287,7 +288,7 @@ __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, 
     for (uint32_t p = threadIdx.x; p < total; p += BS) {
       const uint32_t jj = upper_index(s_pre, BS, p);
       uint32_t kth = p - (jj ? s_pre[jj - 1] : 0u);
-      const unsigned long long* ri = rows + (size_t)(part + (c0 + jj) * n_parts) * W;
+      const unsigned long long* ri = rows + (size_t)(part + (c0 + jj) * n_parts) * RS;
       if (MAXW > 0) {
         uint64_t wi[MAXW > 0 ? MAXW : 1];
 #pragma unroll
@@ -305,7 +306,7 @@ __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, 
             kth -= pw;
           }
         }
-        const unsigned long long* rj = rows + (size_t)j * W;
+        const unsigned long long* rj = rows + (size_t)j * RS;
         uint64_t m[MAXW > 0 ? MAXW : 1];
 #pragma unroll
         for (int x = 0; x < MAXW; ++x) m[x] = wi[x] & rj[x];
@@ -321,7 +322,7 @@ __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, 
               mm &= mm - 1;
               // Row l holds only positions > l (rows are upper triangular),
               // so its words below x are zero and are not read.
-              const unsigned long long* rl = rows + (size_t)l * W;
+              const unsigned long long* rl = rows + (size_t)l * RS;
               uint64_t a[MAXW > 0 ? MAXW : 1];
 #pragma unroll
               for (int y = x; y < MAXW; ++y) a[y] = rl[y];
@@ -339,7 +340,7 @@ __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, 
           ++w;
         }
         const uint32_t j = w * 64 + select_bit(ri[w], kth);
-        const unsigned long long* rj = rows + (size_t)j * W;
+        const unsigned long long* rj = rows + (size_t)j * RS;
         if (K == 4) {
           for (uint32_t x = 0; x < W; ++x) c += __popcll(ri[x] & rj[x]);
         } else {
@@ -348,7 +349,7 @@ __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, 
             while (mm) {
               const uint32_t l = x * 64 + ctz64(mm);
               mm &= mm - 1;
-              const unsigned long long* rl = rows + (size_t)l * W;
+              const unsigned long long* rl = rows + (size_t)l * RS;
               for (uint32_t y = x; y < W; ++y) c += __popcll(ri[y] & rj[y] & rl[y]);
             }
           }
@@ -362,11 +363,11 @@ __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, 
 
 template <int K, int MODE, int BS>
 __device__ __forceinline__ uint64_t count_rows(const unsigned long long* rows, uint32_t d,
-                                               uint32_t W, uint32_t part, uint32_t n_parts,
-                                               const uint32_t* lens) {
+                                               uint32_t W, uint32_t RS, uint32_t part,
+                                               uint32_t n_parts, const uint32_t* lens) {
   uint64_t c = 0;
   for (uint32_t i = part + threadIdx.x * n_parts; i < d; i += BS * n_parts) {
-    const unsigned long long* ri = rows + (size_t)i * W;
+    const unsigned long long* ri = rows + (size_t)i * RS;
     if (MODE == kCount) {
       for (uint32_t w = 0; w < W; ++w) c += __popcll(ri[w]);  // K == 3 only
     } else {
@@ -382,7 +383,7 @@ __device__ __forceinline__ uint64_t count_rows(const unsigned long long* rows, u
             if (K == 4) {
               e += w1 + lens[j];
             } else {
-              const unsigned long long* rj = rows + (size_t)j * W;
+              const unsigned long long* rj = rows + (size_t)j * RS;
               uint64_t w2 = 0;
               for (uint32_t x = 0; x < W; ++x) w2 += __popcll(ri[x] & rj[x]);
               e += w1 + lens[j] + w2;
@@ -422,12 +423,17 @@ __global__ __launch_bounds__(BS) void clique_block_kernel(
   __shared__ int s_any_probe;
   __shared__ unsigned long long s_red[(BS / kWave)];
   const uint32_t Wcap = (cap + 63) / 64;
+  // LDS rows of the 8-word tier are padded to an odd stride (in 8-byte words):
+  // the rows read together sit at data-dependent indices, and an even stride
+  // maps them onto few banks.
+  constexpr bool kPad = LDS_ROWS && MAXW == 8;
+  const uint32_t RScap = kPad ? (Wcap | 1u) : Wcap;
   unsigned long long* rows;
   uint32_t *lens, *begs, *keys;
   uint32_t* vals;
   if (LDS_ROWS) {
     rows = s_dyn;
-    lens = reinterpret_cast<uint32_t*>(rows + (size_t)cap * Wcap);
+    lens = reinterpret_cast<uint32_t*>(rows + (size_t)cap * RScap);
     begs = lens + cap;
     keys = begs + cap;
     vals = keys + table;
@@ -452,6 +458,7 @@ __global__ __launch_bounds__(BS) void clique_block_kernel(
     const uint32_t sb = uniform(off[u]), d = uniform(off[u + 1]) - sb;
     const uint32_t* S = adj + sb;
     const uint32_t W = (d + 63) / 64;
+    const uint32_t RS = kPad ? (W | 1u) : W;
     if (threadIdx.x == 0) s_any_probe = 0;
     for (uint32_t i = threadIdx.x; i < d; i += BS) {
       const uint32_t v = S[i];
@@ -460,7 +467,7 @@ __global__ __launch_bounds__(BS) void clique_block_kernel(
       lens[i] = len;
       begs[i] = b;
     }
-    for (uint32_t i = threadIdx.x; i < d * W; i += BS) rows[i] = 0ull;
+    for (uint32_t i = threadIdx.x; i < d * RS; i += BS) rows[i] = 0ull;
     if (LDS_ROWS)
       for (uint32_t i = threadIdx.x; i < table; i += BS) keys[i] = kEmpty;
     __syncthreads();
@@ -508,7 +515,7 @@ __global__ __launch_bounds__(BS) void clique_block_kernel(
             const uint32_t p = upper_index(S, d, x);
             idx = (p > 0 && S[p - 1] == x) ? (int)(p - 1) : -1;
           }
-          if (idx >= 0) atomicOr(&rows[(size_t)row_i * W + (idx >> 6)], 1ull << (idx & 63));
+          if (idx >= 0) atomicOr(&rows[(size_t)row_i * RS + (idx >> 6)], 1ull << (idx & 63));
         }
       }
       __syncthreads();
@@ -530,7 +537,7 @@ __global__ __launch_bounds__(BS) void clique_block_kernel(
           const uint32_t k = f / d, t = f - k * d;
           const uint32_t ri = s_pre[k];
           if (sorted_contains(adj + begs[ri], lens[ri], S[t]))
-            atomicOr(&rows[(size_t)ri * W + (t >> 6)], 1ull << (t & 63));
+            atomicOr(&rows[(size_t)ri * RS + (t >> 6)], 1ull << (t & 63));
         }
         __syncthreads();
       }
@@ -541,15 +548,15 @@ __global__ __launch_bounds__(BS) void clique_block_kernel(
     if (MODE == kCount && K >= 4) {
       // Narrowest register width that holds this root's rows.
       if (MAXW == 8 && W <= 2)
-        c = count_pairs<K, (MAXW == 8 ? 2 : MAXW), BS>(rows, d, W, part, n_parts, s_pre, s_wsum);
+        c = count_pairs<K, (MAXW == 8 ? 2 : MAXW), BS>(rows, d, W, RS, part, n_parts, s_pre, s_wsum);
       else if (MAXW == 8 && W <= 4)
-        c = count_pairs<K, (MAXW == 8 ? 4 : MAXW), BS>(rows, d, W, part, n_parts, s_pre, s_wsum);
+        c = count_pairs<K, (MAXW == 8 ? 4 : MAXW), BS>(rows, d, W, RS, part, n_parts, s_pre, s_wsum);
       else if (MAXW == 16 && W <= 12)
-        c = count_pairs<K, (MAXW == 16 ? 12 : MAXW), BS>(rows, d, W, part, n_parts, s_pre, s_wsum);
+        c = count_pairs<K, (MAXW == 16 ? 12 : MAXW), BS>(rows, d, W, RS, part, n_parts, s_pre, s_wsum);
       else
-        c = count_pairs<K, MAXW, BS>(rows, d, W, part, n_parts, s_pre, s_wsum);
+        c = count_pairs<K, MAXW, BS>(rows, d, W, RS, part, n_parts, s_pre, s_wsum);
     } else {
-      c = count_rows<K, MODE, BS>(rows, d, W, part, n_parts, lens);
+      c = count_rows<K, MODE, BS>(rows, d, W, RS, part, n_parts, lens);
     }
     if (MODE == kWork) {
       c = wave_sum_u64(c);
@@ -583,9 +590,9 @@ uint32_t table_for(uint32_t cap) {
   return table;
 }
 
-size_t block_lds_bytes(uint32_t cap, uint32_t table) {
+size_t block_lds_bytes(uint32_t cap, uint32_t table, bool pad) {
   const uint32_t W = (cap + 63) / 64;
-  return (size_t)cap * W * 8 + (size_t)cap * 8 + (size_t)table * 8;
+  return (size_t)cap * (pad ? (W | 1u) : W) * 8 + (size_t)cap * 8 + (size_t)table * 8;
 }
 
 // Largest out-degree whose rows fit one block's LDS on this device (at most
@@ -600,7 +607,7 @@ uint32_t lds_wide_cap() {
       limit = prop.sharedMemPerBlock;
     uint32_t cap = kBlockCap;
     for (uint32_t c = 1024; c > kBlockCap; c -= 64) {
-      if (block_lds_bytes(c, table_for(c)) + 4096 <= limit) {
+      if (block_lds_bytes(c, table_for(c), false) + 4096 <= limit) {
         cap = c;
         break;
       }
@@ -612,8 +619,8 @@ uint32_t lds_wide_cap() {
 
 // Blocks of an LDS-rows tier of this cap resident per CU: limited by LDS
 // (160 KiB per CU) and by 32 waves per CU.
-int lds_blocks_per_cu(uint32_t cap, int block_threads) {
-  const size_t lds = block_lds_bytes(cap, table_for(cap));
+int lds_blocks_per_cu(uint32_t cap, int block_threads, bool pad) {
+  const size_t lds = block_lds_bytes(cap, table_for(cap), pad);
   const int by_waves = std::max(1, 32 / (block_threads / kWave));
   return std::min(by_waves, std::max<int>(1, (int)(160 * 1024 / (lds + 2048))));
 }
@@ -626,13 +633,13 @@ int launch_lds_tier(gpm_graph* g, hipStream_t stream, const uint32_t* roots,
                     const char* name) {
   const int cus = device_cus(g);
   const uint32_t table = table_for(cap);
-  const size_t lds = block_lds_bytes(cap, table);
+  const size_t lds = block_lds_bytes(cap, table, MAXW == 8);
   auto kern = clique_block_kernel<K, MODE, true, MAXW, BS>;
   if (lds > 48 * 1024)
     GPM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const unsigned blocks =
-      (unsigned)std::min<uint64_t>(n_roots, (uint64_t)cus * lds_blocks_per_cu(cap, BS));
+      (unsigned)std::min<uint64_t>(n_roots, (uint64_t)cus * lds_blocks_per_cu(cap, BS, MAXW == 8));
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(BS), lds, stream, g->toff, g->tadj, roots,
                      parts, n_roots, cap, table, queue, nullptr, nullptr, out, work);
   GPM_HIP(hipGetLastError());
@@ -773,8 +780,8 @@ int gpm_clique(gpm_graph* g, int k, const gpm_sched* sched, uint64_t* out) {
   if (req.policy == GPM_SCHED_LEARNED) {
     req.heavy_cost = learned_heavy_cost(req.cost, g->n);
     const uint32_t cus = (uint32_t)device_cus(g);
-    req.resident_block = cus * lds_blocks_per_cu(req.block_cap, kLdsBlock);
-    req.resident_block2 = cus * lds_blocks_per_cu(req.block2_cap, kWideBlock);
+    req.resident_block = cus * lds_blocks_per_cu(req.block_cap, kLdsBlock, true);
+    req.resident_block2 = cus * lds_blocks_per_cu(req.block2_cap, kWideBlock, false);
     req.resident_big = cus;
   }
   const RootTiers* t = nullptr;
