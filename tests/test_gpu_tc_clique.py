@@ -198,3 +198,24 @@ def test_probe_paths_closed_form(gpm):
             deg = np.diff(gpm.orient(g).offsets.astype(np.int64)).astype(np.float32)
             assert dg.clique(k, policy=gpm.SCHED_LEARNED, cost=deg ** 2) == want
         assert dg.tc(gpm.TC_EDGE) == comb(h, 3) + sum(comb(t, 2) for t in ts)
+
+
+def test_empty_and_tiny_graphs(gpm):
+    """Edge cases: no edges, a single vertex, isolated vertices, one edge."""
+    cases = [
+        (gpm.HostCSR(np.array([0], dtype=np.uint32), np.zeros(0, dtype=np.uint32)), 0),
+        (gpm.HostCSR(np.array([0, 0], dtype=np.uint32), np.zeros(0, dtype=np.uint32)), 0),
+        (gpm.HostCSR(np.zeros(101, dtype=np.uint32), np.zeros(0, dtype=np.uint32)), 0),
+        (gpm.csr_from_edges(5, [0], [1], gpm.CSR_CLEAN), 0),
+        (graphs.complete(3), 1),
+    ]
+    for g, tri in cases:
+        with gpm.DeviceGraph(gpm.orient(g), oriented=True) as dg:
+            assert dg.tc(gpm.TC_EDGE) == tri
+            assert dg.tc(gpm.TC_VERTEX) == tri
+            for k in (3, 4, 5):
+                assert dg.clique(k) == (tri if k == 3 else 0)
+            assert dg.tc_repeat(gpm.TC_EDGE, 40) == tri
+        with gpm.DeviceGraph(g, oriented=False) as sg:
+            counts, _ = sg.pattern("Motif4", gpm.FORM_PATTERN)
+            assert counts == [0, 0, 0, 0, 0, 0]
