@@ -1,16 +1,21 @@
 """Benchmark: k-clique counting (k = 4 and 5) on a seeded R-MAT graph.
 
-Workload (one "step"): count 4-cliques and 5-cliques of R-MAT scale 20,
-edge factor 16 (a, b, c, d) = (.57, .19, .19, .05), symmetrised, deduplicated,
-degree-oriented; the oriented CSR is resident in HBM before timing starts.
-value = oriented edges processed per second over the step (each k pass
-roots at every oriented edge once): 2 |E+| / step time.
+Workload (one "step"): the 4-clique and 5-clique counts of R-MAT scale 20,
+edge factor 16 (a, b, c, d) = (.57, .19, .19, .05), symmetrised,
+deduplicated, degree-oriented; the oriented CSR is resident in HBM before
+timing starts.  Both counts come from one fused pass over the roots
+(gpm_clique_all); the two single-k passes are timed beside it for reference.
+value = 2 |E+| / step time: oriented edges per second, each of the two
+counts rooting at every oriented edge once (the reference's edge-rooted
+CFSolver runs one pass per k).
 
 Also reported: event-timed per-kernel split, the roofline of the dominant
 kernel (algorithmic bytes of the reference's edge-rooted plan for the roots
-that kernel serves, over its mean duration), and a CPU baseline: the oracle
-timed on a bounded sample of roots on this host, whose counts must equal
-the engine's counts on the same roots.
+that kernel serves, over its mean duration), the measured HBM traffic of that
+kernel from the committed profiler summary, a CPU baseline (the oracle timed
+on a bounded sample of roots on this host, whose counts must equal the
+engine's on the same roots) and extra workloads (triangle counting, learned
+vs plain scheduling, labeled matching).
 """
 import argparse
 import json
@@ -261,32 +266,44 @@ def main():
     deg = d.degrees()
     dg = G.DeviceGraph(d, oriented=True)
     ks = (4, 5)
+    # One step = the 4- and 5-clique counts of every root, from one fused
+    # pass (gpm_clique_all builds each root's rows once and counts k = 3, 4
+    # and 5 from them).
     counts = {}
     for _ in range(max(args.warmup, 1)):
-        for k in ks:
-            counts[k] = dg.clique(k)
+        c3, counts[4], counts[5] = dg.clique_all()
 
     # Timed region: every call is synchronous (ends with a stream sync).
     kern_ms = {}
     kern_launches = {}
-    call_ms = {k: 0.0 for k in ks}
+    call_ms = 0.0
     t_start = time.perf_counter()
     for _ in range(args.steps):
-        for k in ks:
-            c = dg.clique(k)
-            assert c == counts[k], "count changed between runs"
-            call_ms[k] += dg.last_kernel_ms()
-            for name, ms in dg.kernel_breakdown():
-                key = f"k{k}:{name}"
-                kern_ms[key] = kern_ms.get(key, 0.0) + ms
-                kern_launches[key] = kern_launches.get(key, 0) + 1
+        got = dg.clique_all()
+        assert got[1:] == (counts[4], counts[5]), "count changed between runs"
+        call_ms += dg.last_kernel_ms()
+        for name, ms in dg.kernel_breakdown():
+            key = f"k45:{name}"
+            kern_ms[key] = kern_ms.get(key, 0.0) + ms
+            kern_launches[key] = kern_launches.get(key, 0) + 1
     wall = time.perf_counter() - t_start
     ms_per_step = wall * 1e3 / args.steps
     value = 2 * d.m * args.steps / wall
 
-    # Roofline of the dominant kernel.
+    # The same counts as two separate passes (one per k), for reference.
+    separate = {}
+    for k in ks:
+        assert dg.clique(k) == counts[k]
+        best = None
+        for _ in range(max(1, min(args.steps, 3))):
+            dg.clique(k)
+            best = dg.last_kernel_ms() if best is None else min(best, dg.last_kernel_ms())
+        separate[f"k{k}_ms"] = best
+
+    # Roofline of the dominant kernel: its roots' bytes under the reference's
+    # k = 5 plan (which contains the k = 4 plan's set operations).
     dom = max(kern_ms, key=kern_ms.get)
-    dom_k = int(dom[1])
+    dom_k = 5
     dom_name = dom.split(":", 1)[1]
     caps = G.clique_tier_caps()
     tiers = tier_of(deg, dom_k, caps)
@@ -312,12 +329,14 @@ def main():
         "data": "synthetic (seeded R-MAT generated on the host)",
         "config": {
             "workload": f"k-clique k=4,5 on R-MAT scale {args.scale}, edge factor 16, "
-                        "a,b,c,d=.57,.19,.19,.05, symmetrised+dedup, degree-oriented DAG",
+                        "a,b,c,d=.57,.19,.19,.05, symmetrised+dedup, degree-oriented DAG; "
+                        "both counts from one fused pass per step",
             "n": d.n, "oriented_edges": d.m, "max_out_degree": int(deg.max()),
             "seed": SEED, "parallelism": "single GPU",
         },
-        "counts": {f"k{k}": int(counts[k]) for k in ks},
-        "matches_per_s": {f"k{k}": counts[k] / (call_ms[k] * 1e-3 / args.steps) for k in ks},
+        "counts": {"k3": int(c3), "k4": int(counts[4]), "k5": int(counts[5])},
+        "matches_per_s": (counts[4] + counts[5]) / (call_ms * 1e-3 / args.steps),
+        "separate_passes_ms": separate,
         "kernel_ms_per_step": {k: v / args.steps for k, v in sorted(kern_ms.items())},
         "roofline": {
             "bound": "hbm",

@@ -52,7 +52,9 @@ constexpr int kWideBlock = 1024;
 constexpr int kHbmBlock = 1024;
 // Consecutive list entries one thread streams in the row build.
 constexpr uint32_t kSeg = 8;
-enum { kCount = 0, kBytes = 1, kWork = 2 };
+enum { kCount = 0, kBytes = 1, kWork = 2, kAll = 3 };
+// kAll (with K = 5): the 3-, 4- and 5-clique counts of every root from one
+// build of its rows; out[0..2] receive them.
 
 // A list N+(S[i]) much longer than S is not streamed: each S[t] is looked up
 // in it by binary search instead (d searches of log2(len) steps against len
@@ -107,7 +109,7 @@ __global__ __launch_bounds__(kBlock) void clique_wave_kernel(
   uint32_t* lens = s_len[w];
   uint32_t* svals = s_sv[w];
   constexpr uint32_t hmask = kWaveTable - 1;
-  uint64_t acc = 0;
+  uint64_t acc = 0, acc3 = 0, acc4 = 0;
   while (true) {
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(queue, chunk);
@@ -162,7 +164,8 @@ __global__ __launch_bounds__(kBlock) void clique_wave_kernel(
       __builtin_amdgcn_wave_barrier();
       const uint64_t row = act ? rows[lane] : 0ull;
       uint64_t c = 0;
-      if (MODE == kCount) {
+      if (MODE == kAll) acc3 += __popcll(row);
+      if (MODE == kCount || MODE == kAll) {
         if (K == 3) {
           c = __popcll(row);
         } else {
@@ -181,6 +184,7 @@ __global__ __launch_bounds__(kBlock) void clique_wave_kernel(
               const uint64_t ri = rows[i];
               const uint32_t j = select_bit(ri, p - excl);
               const uint64_t mij = ri & rows[j];
+              if (MODE == kAll) acc4 += __popcll(mij);
               if (K == 4) {
                 c += __popcll(mij);
               } else {
@@ -229,7 +233,16 @@ __global__ __launch_bounds__(kBlock) void clique_wave_kernel(
       __builtin_amdgcn_wave_barrier();
     }
   }
-  if (MODE != kWork) {
+  if (MODE == kAll) {
+    acc3 = wave_sum_u64(acc3);
+    acc4 = wave_sum_u64(acc4);
+    acc = wave_sum_u64(acc);
+    if (lane == 0) {
+      if (acc3) atomicAdd(out, (unsigned long long)acc3);
+      if (acc4) atomicAdd(out + 1, (unsigned long long)acc4);
+      if (acc) atomicAdd(out + 2, (unsigned long long)acc);
+    }
+  } else if (MODE != kWork) {
     acc = wave_sum_u64(acc);
     if (lane == 0 && acc) atomicAdd(out, (unsigned long long)acc);
   }
@@ -258,11 +271,11 @@ __device__ __forceinline__ uint32_t block_incl_scan(uint32_t v, uint32_t* s_wsum
 // one wait covers them (words past W belong to the next row or to the tail
 // of the block's LDS, and are masked out by m[x] = 0 for x >= W).
 // MAXW == 0 is the generic path for rows in HBM.
-template <int K, int MAXW, int BS>
+template <int K, int MAXW, int BS, bool ALL = false>
 __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, uint32_t d,
                                                 uint32_t W, uint32_t RS, uint32_t part,
                                                 uint32_t n_parts, uint32_t* s_pre,
-                                                uint32_t* s_wsum) {
+                                                uint32_t* s_wsum, uint64_t& c3, uint64_t& c4) {
   uint64_t c = 0;
   // This task counts rows part, part + n_parts, ...: nr of them.
   const uint32_t nr = d > part ? (d - part + n_parts - 1) / n_parts : 0u;
@@ -281,6 +294,7 @@ __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, 
         for (uint32_t w = 0; w < W; ++w) pc += (uint32_t)__popcll(r0[w]);
       }
     }
+    if (ALL) c3 += pc;
     const uint32_t incl = block_incl_scan(pc, s_wsum);
     s_pre[threadIdx.x] = incl;
     __syncthreads();
@@ -314,6 +328,9 @@ __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, 
 #pragma unroll
           for (int x = 0; x < MAXW; ++x) c += __popcll(m[x]);
         } else {
+          if (ALL)
+#pragma unroll
+            for (int x = 0; x < MAXW; ++x) c4 += __popcll(m[x]);
 #pragma unroll
           for (int x = 0; x < MAXW; ++x) {
             uint64_t mm = m[x];
@@ -346,6 +363,7 @@ __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, 
         } else {
           for (uint32_t x = 0; x < W; ++x) {
             uint64_t mm = ri[x] & rj[x];
+            if (ALL) c4 += __popcll(mm);
             while (mm) {
               const uint32_t l = x * 64 + ctz64(mm);
               mm &= mm - 1;
@@ -445,7 +463,7 @@ __global__ __launch_bounds__(BS) void clique_block_kernel(
     vals = nullptr;
   }
   const uint32_t hmask = table - 1;
-  uint64_t acc = 0;
+  uint64_t acc = 0, acc3 = 0, acc4 = 0;
   while (true) {
     if (threadIdx.x == 0) s_root = atomicAdd(queue, 1u);
     __syncthreads();
@@ -545,16 +563,21 @@ __global__ __launch_bounds__(BS) void clique_block_kernel(
     __syncthreads();
     if (!LDS_ROWS) __threadfence();
     uint64_t c;
-    if (MODE == kCount && K >= 4) {
+    if ((MODE == kCount || MODE == kAll) && K >= 4) {
       // Narrowest register width that holds this root's rows.
+      constexpr bool ALL = MODE == kAll;
       if (MAXW == 8 && W <= 2)
-        c = count_pairs<K, (MAXW == 8 ? 2 : MAXW), BS>(rows, d, W, RS, part, n_parts, s_pre, s_wsum);
+        c = count_pairs<K, (MAXW == 8 ? 2 : MAXW), BS, ALL>(rows, d, W, RS, part, n_parts, s_pre,
+                                                            s_wsum, acc3, acc4);
       else if (MAXW == 8 && W <= 4)
-        c = count_pairs<K, (MAXW == 8 ? 4 : MAXW), BS>(rows, d, W, RS, part, n_parts, s_pre, s_wsum);
+        c = count_pairs<K, (MAXW == 8 ? 4 : MAXW), BS, ALL>(rows, d, W, RS, part, n_parts, s_pre,
+                                                            s_wsum, acc3, acc4);
       else if (MAXW == 16 && W <= 12)
-        c = count_pairs<K, (MAXW == 16 ? 12 : MAXW), BS>(rows, d, W, RS, part, n_parts, s_pre, s_wsum);
+        c = count_pairs<K, (MAXW == 16 ? 12 : MAXW), BS, ALL>(rows, d, W, RS, part, n_parts,
+                                                              s_pre, s_wsum, acc3, acc4);
       else
-        c = count_pairs<K, MAXW, BS>(rows, d, W, RS, part, n_parts, s_pre, s_wsum);
+        c = count_pairs<K, MAXW, BS, ALL>(rows, d, W, RS, part, n_parts, s_pre, s_wsum, acc3,
+                                          acc4);
     } else {
       c = count_rows<K, MODE, BS>(rows, d, W, RS, part, n_parts, lens);
     }
@@ -572,7 +595,16 @@ __global__ __launch_bounds__(BS) void clique_block_kernel(
     }
     __syncthreads();
   }
-  if (MODE != kWork) {
+  if (MODE == kAll) {
+    acc3 = wave_sum_u64(acc3);
+    acc4 = wave_sum_u64(acc4);
+    acc = wave_sum_u64(acc);
+    if (lane_id() == 0) {
+      if (acc3) atomicAdd(out, (unsigned long long)acc3);
+      if (acc4) atomicAdd(out + 1, (unsigned long long)acc4);
+      if (acc) atomicAdd(out + 2, (unsigned long long)acc);
+    }
+  } else if (MODE != kWork) {
     acc = wave_sum_u64(acc);
     if (lane_id() == 0 && acc) atomicAdd(out, (unsigned long long)acc);
   }
@@ -643,7 +675,7 @@ int launch_lds_tier(gpm_graph* g, hipStream_t stream, const uint32_t* roots,
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(BS), lds, stream, g->toff, g->tadj, roots,
                      parts, n_roots, cap, table, queue, nullptr, nullptr, out, work);
   GPM_HIP(hipGetLastError());
-  if (MODE == kCount && stream == g->stream) timing_mark(g, name);
+  if ((MODE == kCount || MODE == kAll) && stream == g->stream) timing_mark(g, name);
   return GPM_OK;
 }
 
@@ -659,7 +691,7 @@ int launch_wave_tier(gpm_graph* g, hipStream_t stream, const RootTiers* t, unsig
   hipLaunchKernelGGL((clique_wave_kernel<K, MODE>), dim3(blocks), dim3(kBlock), 0, stream,
                      g->toff, g->tadj, t->d_wave, t->n_wave, chunk, q, out, work);
   GPM_HIP(hipGetLastError());
-  if (MODE == kCount && stream == g->stream) timing_mark(g, "clique_wave");
+  if ((MODE == kCount || MODE == kAll) && stream == g->stream) timing_mark(g, "clique_wave");
   return GPM_OK;
 }
 
@@ -696,7 +728,7 @@ int launch_tiers(gpm_graph* g, const RootTiers* t, unsigned long long* out,
                        dim3(kHbmBlock), 0, side, g->toff, g->tadj, t->d_big, t->d_big_parts,
                        t->n_big, cap, 0u, q + 2, grows, gmeta, out, work);
     GPM_HIP(hipGetLastError());
-    if (MODE == kCount && !overlap) timing_mark(g, "clique_hbm_block");
+    if ((MODE == kCount || MODE == kAll) && !overlap) timing_mark(g, "clique_hbm_block");
   }
   if (overlap && t->n_wave) {
     int st = launch_wave_tier<K, MODE>(g, side, t, q, out, work);
@@ -728,6 +760,7 @@ int launch_tiers(gpm_graph* g, const RootTiers* t, unsigned long long* out,
 template <int MODE>
 int launch_k(gpm_graph* g, int k, const RootTiers* t, unsigned long long* out,
              unsigned long long* work, bool overlap = false) {
+  if (MODE == kAll) return launch_tiers<5, MODE>(g, t, out, work, overlap);
   switch (k) {
     case 3: return launch_tiers<3, MODE>(g, t, out, work, overlap);
     case 4: return launch_tiers<4, MODE>(g, t, out, work, overlap);
@@ -798,6 +831,39 @@ int gpm_clique(gpm_graph* g, int k, const gpm_sched* sched, uint64_t* out) {
   GPM_HIP(hipMemcpyAsync(&h, g->counters, sizeof(h), hipMemcpyDeviceToHost, g->stream));
   GPM_HIP(hipStreamSynchronize(g->stream));
   *out = h;
+  return GPM_OK;
+}
+
+int gpm_clique_all(gpm_graph* g, const gpm_sched* sched, uint64_t out[3]) {
+  clear_error();
+  GPM_CHECK_ARG(g && out, "null pointer");
+  GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "k-clique counting needs an ORIENTED graph");
+  GPM_CHECK_ARG(!sched || sched->policy == GPM_SCHED_DEGREE || sched->policy == GPM_SCHED_LEARNED,
+                "unknown scheduling policy");
+  GPM_CHECK_ARG(!sched || sched->policy != GPM_SCHED_LEARNED || sched->root_cost,
+                "LEARNED policy needs root_cost");
+  int st = ensure_topo(g);
+  if (st != GPM_OK) return st;
+  TierRequest req = request_from(sched, 2u);  // triangles need out-degree >= 2
+  req.rank = g->h_rank.data();
+  if (req.policy == GPM_SCHED_LEARNED) {
+    req.heavy_cost = learned_heavy_cost(req.cost, g->n);
+    const uint32_t cus = (uint32_t)device_cus(g);
+    req.resident_block = cus * lds_blocks_per_cu(req.block_cap, kLdsBlock, true);
+    req.resident_block2 = cus * lds_blocks_per_cu(req.block2_cap, kWideBlock, false);
+    req.resident_big = cus;
+  }
+  const RootTiers* t = nullptr;
+  if ((st = get_tiers(g, req, &t)) != GPM_OK) return st;
+  if ((st = timing_begin(g)) != GPM_OK) return st;
+  if ((st = zero_counters(g, 3)) != GPM_OK) return st;
+  const bool overlap = req.policy == GPM_SCHED_LEARNED;
+  if ((st = launch_k<kAll>(g, 5, t, g->counters, nullptr, overlap)) != GPM_OK) return st;
+  if ((st = timing_end(g)) != GPM_OK) return st;
+  unsigned long long h[3] = {0, 0, 0};
+  GPM_HIP(hipMemcpyAsync(h, g->counters, sizeof(h), hipMemcpyDeviceToHost, g->stream));
+  GPM_HIP(hipStreamSynchronize(g->stream));
+  for (int i = 0; i < 3; ++i) out[i] = h[i];
   return GPM_OK;
 }
 

@@ -287,6 +287,17 @@ class DeviceGraph:
         _check(lib().gpm_clique(self._h, C.c_int(k), C.byref(s), C.byref(out)))
         return out.value
 
+    def clique_all(self, policy: int = SCHED_DEGREE, cost: Optional[np.ndarray] = None):
+        """(k3, k4, k5) clique counts from one pass over the roots."""
+        s = Sched()
+        s.policy = policy
+        if cost is not None:
+            self._cost = np.ascontiguousarray(cost, dtype=np.float32)
+            s.root_cost = _ptr(self._cost, C.c_float)
+        out = (C.c_uint64 * 3)()
+        _check(lib().gpm_clique_all(self._h, C.byref(s), out))
+        return tuple(int(x) for x in out)
+
     def clique_plan_bytes(self, k: int) -> int:
         out = C.c_uint64()
         _check(lib().gpm_clique_plan_bytes(self._h, C.c_int(k), C.byref(out)))

@@ -208,6 +208,14 @@ int gpm_tc_repeat(gpm_graph* g, int variant, int reps, uint64_t* out);
 int gpm_clique(gpm_graph* g, int k, const gpm_sched* sched, uint64_t* out);
 
 /*
+ * 3-, 4- and 5-clique counts of an ORIENTED graph in one pass over the roots
+ * (each root's local DAG is built once and counted for every k):
+ * out[0], out[1], out[2] = counts for k = 3, 4, 5.  No counterpart in the
+ * reference, whose CFSolver counts one k per call (CFSolver.cuh:99-157).
+ */
+int gpm_clique_all(gpm_graph* g, const gpm_sched* sched, uint64_t out[3]);
+
+/*
  * Out-degree limits of the k-clique kernels on this device: roots with
  * out-degree <= *wave_cap run on one wave (rows in registers), <= *lds_cap
  * on one block with LDS rows, <= *wide_cap on one block with wide LDS rows,

@@ -219,3 +219,22 @@ def test_empty_and_tiny_graphs(gpm):
         with gpm.DeviceGraph(g, oriented=False) as sg:
             counts, _ = sg.pattern("Motif4", gpm.FORM_PATTERN)
             assert counts == [0, 0, 0, 0, 0, 0]
+
+
+def test_clique_all_matches_single_k(gpm, orc):
+    """The fused k = 3, 4, 5 pass equals the per-k counts (every tier,
+    both policies, probe paths, K_n closed forms)."""
+    graphs_ = [gpm.orient(gpm.rmat_graph(14, 16, 31)), gpm.orient(graphs.complete(300)),
+               gpm.orient(graphs.complete(1100)), gpm.orient(graphs.turan(130, 6))]
+    for d in graphs_:
+        with gpm.DeviceGraph(d, oriented=True) as dg:
+            want = tuple(dg.clique(k) for k in (3, 4, 5)) if d.n != 1100 else \
+                (comb(1100, 3), comb(1100, 4), comb(1100, 5))
+            got = dg.clique_all()
+            assert got == want
+            deg = d.degrees().astype(np.float32)
+            assert dg.clique_all(gpm.SCHED_LEARNED, deg ** 3) == got
+    g = gpm.rmat_graph(12, 16, 1012)
+    d = gpm.orient(g)
+    with gpm.DeviceGraph(d, oriented=True) as dg:
+        assert dg.clique_all() == tuple(orc.clique(d.offsets, d.adj, k) for k in (3, 4, 5))
