@@ -193,12 +193,15 @@ def extra_match(scale):
         for name, kw in (("plain", dict(autotune=1, policy=G.SCHED_DEGREE)),
                          ("learned", dict(autotune=1, policy=G.SCHED_LEARNED)),
                          ("graph_replay", dict(autotune=0, chunk=8, use_graph=1))):
+            t0 = time.perf_counter()
             c = dg.match_batch(qs, **kw)
+            first = (time.perf_counter() - t0) * 1e3  # planning, trials, roots, launch
             ms = []
             for _ in range(3):
                 assert np.array_equal(dg.match_batch(qs, **kw), c)
                 ms.append(dg.last_kernel_ms())
-            res[name] = {"ms": min(ms), "exact_vs_reference": bool(np.array_equal(c, want)),
+            res[name] = {"ms": min(ms), "first_call_ms": first,
+                         "exact_vs_reference": bool(np.array_equal(c, want)),
                          "embeddings_per_s": float(c.sum()) / (min(ms) * 1e-3)}
         res["embeddings"] = int(want.sum())
         res["learned_speedup"] = res["plain"]["ms"] / res["learned"]["ms"]
@@ -211,8 +214,12 @@ def extra_match(scale):
         for qname in ("triangle", "4-cycle", "diamond", "three-star"):
             row = {}
             for pol, pname in ((G.SCHED_DEGREE, "plain"), (G.SCHED_LEARNED, "learned")):
+                t0 = time.perf_counter()
                 c = dg.match_batch([named[qname]], autotune=1, policy=pol)
-                row[pname] = {"count": int(c[0]), "ms": dg.last_kernel_ms()}
+                first = (time.perf_counter() - t0) * 1e3
+                c2 = dg.match_batch([named[qname]], autotune=1, policy=pol)
+                assert int(c2[0]) == int(c[0])
+                row[pname] = {"count": int(c[0]), "ms": dg.last_kernel_ms(), "first_call_ms": first}
             row["identical"] = row["plain"]["count"] == row["learned"]["count"]
             res[qname] = row
             print(f"[bench] {qname} done", file=sys.stderr, flush=True)

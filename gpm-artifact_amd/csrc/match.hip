@@ -11,7 +11,11 @@
 //    from the data graph's label statistics -- for every start vertex the
 //    order grows by the vertex that keeps the expected number of partial
 //    embeddings smallest, and the start with the least expected total work
-//    wins.  The same estimate sizes the autotuner's chunks.
+//    wins.  The same estimate sizes the autotuner's chunks.  With the
+//    autotuner on, a query whose cost-based order differs from the BFS
+//    order gets both plans run on a sample of their roots (its heaviest
+//    roots plus a hash-selected share of the rest) in one trial launch that
+//    counts wave steps, and keeps the plan with fewer estimated steps.
 //  * Candidate filter at every depth: right label, enough degree and, per
 //    neighbour label, enough neighbours of that label (the NLF filter of
 //    myceci.cpp:120-169), read in O(1) per label from the label-partitioned
@@ -29,11 +33,12 @@
 //  * Autotuner (host): the work one root of a query costs is estimated from
 //    the data graph's label statistics (expected label-slice lengths and
 //    pass rates per depth of the plan); each query's chunk (roots per task)
-//    is set so a task costs about the same, and queries are laid out
-//    heaviest first.
-//  * The tuned launch (counter reset + enumeration) is captured once into a
-//    hipGraph and replayed.
+//    is set so a task costs about the same (capped so the batch keeps two
+//    tasks per resident wave), and queries are laid out heaviest first.
+//  * With a fixed chunk, the launch (counter reset + enumeration) can be
+//    built once as a two-node hipGraph and replayed.
 #include <algorithm>
+#include <cstring>
 #include <numeric>
 #include <queue>
 #include <vector>
@@ -213,7 +218,7 @@ __device__ __forceinline__ bool passes_tail(const DevGraph& g, const QPlan& p, c
 // |A_B| for a subset B is counted while streaming the candidates of B's
 // first member (each x tested against the later members' full filters).
 // Arithmetic is modulo 2^64, exact when the true count fits.
-__device__ uint64_t count_tail(const DevGraph& g, const QPlan& p, WaveState& s) {
+__device__ uint64_t count_tail(const DevGraph& g, const QPlan& p, WaveState& s, uint64_t& steps) {
   const uint32_t lane = lane_id();
   uint64_t prod = 1;
   for (uint32_t gi = 0; gi < p.n_groups; ++gi) {
@@ -224,6 +229,7 @@ __device__ uint64_t count_tail(const DevGraph& g, const QPlan& p, WaveState& s) 
       init_depth(g, p, s, d);
       const uint32_t sl = uniform(s.sl[d]), sb = uniform(s.sb[d]);
       for (uint32_t cur = 0; cur < sl; cur += kWave) {
+        ++steps;
         const uint32_t i = cur + lane;
         uint32_t mask = 0;
         if (i < sl) {
@@ -257,18 +263,19 @@ __device__ uint64_t count_tail(const DevGraph& g, const QPlan& p, WaveState& s) 
 // of the entries [lo1, hi1) of the depth-1 slice (all: 0, ~0u): a DFS over
 // depths 1..tail-1, each leaf of which adds the tail count.
 __device__ uint64_t count_root(const DevGraph& g, const QPlan& p, WaveState& s, uint32_t v0,
-                               uint32_t lo1, uint32_t hi1) {
+                               uint32_t lo1, uint32_t hi1, uint64_t& steps) {
   const uint32_t lane = lane_id();
   const uint32_t n = p.n;
   if (n == 1) return 1;
   s.emb[0] = v0;
-  if (p.tail == 1) return count_tail(g, p, s);
+  if (p.tail == 1) return count_tail(g, p, s, steps);
   uint64_t total = 0;
   init_depth(g, p, s, 1);
   s.cur[1] = lo1;
   s.sl[1] = min(s.sl[1], hi1);
   int d = 1;
   while (d >= 1) {
+    ++steps;
     const uint32_t dd = (uint32_t)d;
     if (s.mask[dd] == 0ull) {
       const uint32_t cur = s.cur[dd];
@@ -288,7 +295,7 @@ __device__ uint64_t count_root(const DevGraph& g, const QPlan& p, WaveState& s, 
     s.mask[dd] = m & (m - 1);
     s.emb[dd] = uniform(g.ladj[s.sb[dd] + s.base[dd] + j]);
     if (dd + 1 == p.tail) {
-      total += count_tail(g, p, s);
+      total += count_tail(g, p, s, steps);
     } else {
       init_depth(g, p, s, dd + 1);
       ++d;
@@ -318,11 +325,14 @@ struct HeavyTask {
 // Persistent enumeration: every wave pulls tasks from one queue until it is
 // drained.  Tasks [0, n_heavy) are parts of heavy roots (heaviest first);
 // the rest are chunks of a query's root list, which skip the heavy roots.
+// When `work` is given, the wave steps (ballot windows and descents) each
+// plan took are added to work[qid] as well.
 __global__ __launch_bounds__(kBlock) void match_kernel(
     DevGraph g, const QPlan* __restrict__ plans, uint32_t nq, uint32_t n_tasks,
     const uint32_t* __restrict__ roots, const uint32_t* __restrict__ rlen,
     const HeavyTask* __restrict__ heavy, uint32_t n_heavy, uint32_t split_len,
-    unsigned int* __restrict__ queue, unsigned long long* __restrict__ counts) {
+    unsigned int* __restrict__ queue, unsigned long long* __restrict__ counts,
+    unsigned long long* __restrict__ work) {
   __shared__ WaveState s_state[kWavesPerBlock];
   WaveState& s = s_state[threadIdx.x / kWave];
   const uint32_t lane = lane_id();
@@ -333,12 +343,13 @@ __global__ __launch_bounds__(kBlock) void match_kernel(
     if (t >= n_tasks) break;
     uint64_t c = 0;
     uint32_t qid;
+    uint64_t steps = 0;
     if (t < n_heavy) {
       const HeavyTask h = heavy[t];
       const QPlan& p = plans[uniform(h.plan)];
       qid = p.qid;
       c = count_root(g, p, s, uniform(roots[p.root_off + uniform(h.root)]), uniform(h.lo),
-                     uniform(h.hi));
+                     uniform(h.hi), steps);
     } else {
       const uint32_t pi = uniform(find_plan(plans, nq, t - n_heavy));
       const QPlan& p = plans[pi];
@@ -347,10 +358,11 @@ __global__ __launch_bounds__(kBlock) void match_kernel(
       const uint32_t r1 = min(r0 + p.chunk, p.n_roots);
       for (uint32_t r = r0; r < r1; ++r) {
         if (p.tail > 1 && uniform(rlen[p.root_off + r]) > split_len) continue;  // a heavy task
-        c += count_root(g, p, s, uniform(roots[p.root_off + r]), 0u, 0xFFFFFFFFu);
+        c += count_root(g, p, s, uniform(roots[p.root_off + r]), 0u, 0xFFFFFFFFu, steps);
       }
     }
     if (lane == 0 && c) atomicAdd(&counts[qid], (unsigned long long)c);
+    if (work && lane == 0 && steps) atomicAdd(&work[qid], (unsigned long long)steps);
   }
 }
 
@@ -603,6 +615,7 @@ struct MatchBatch {
   uint32_t* d_roots = nullptr;
   uint32_t* d_rlen = nullptr;
   unsigned long long* d_counts = nullptr;
+  unsigned long long* d_work = nullptr;  // wave steps per counter (plan trials only)
   unsigned int* d_rootcnt = nullptr;
   unsigned int* d_queue = nullptr;
   HeavyTask* d_heavy = nullptr;  // own allocation
@@ -652,7 +665,7 @@ int launch_batch(gpm_graph* g, MatchBatch* b) {
                      b->d_counts, b->n_queries, b->d_queue, 4);
   hipLaunchKernelGGL(match_kernel, dim3(b->grid), dim3(kBlock), 0, g->stream, b->dg,
                      b->d_plans, b->nq, b->n_tasks, b->d_roots, b->d_rlen, b->d_heavy,
-                     b->n_heavy, b->split_len, b->d_queue, b->d_counts);
+                     b->n_heavy, b->split_len, b->d_queue, b->d_counts, b->d_work);
   GPM_HIP(hipGetLastError());
   return GPM_OK;
 }
@@ -674,7 +687,7 @@ int build_graph(MatchBatch* b) {
   GPM_HIP(hipGraphAddKernelNode(&zn, b->graph, nullptr, 0, &zp));
   void* margs[] = {&b->dg,      &b->d_plans,   &b->nq,      &b->n_tasks,
                    &b->d_roots, &b->d_rlen,    &b->d_heavy, &b->n_heavy,
-                   &b->split_len, &b->d_queue, &b->d_counts};
+                   &b->split_len, &b->d_queue, &b->d_counts, &b->d_work};
   hipKernelNodeParams mp{};
   mp.func = reinterpret_cast<void*>(&match_kernel);
   mp.gridDim = dim3(b->grid);
@@ -687,90 +700,192 @@ int build_graph(MatchBatch* b) {
   return GPM_OK;
 }
 
-// Plans, filters roots, tunes and lays out a batch.
-int prepare_batch(gpm_graph* g, const gpm_query* queries, int n_queries, const gpm_tuning& tun,
-                  MatchBatch* b) {
-  std::vector<QPlan> plans;
-  for (int qi = 0; qi < n_queries; ++qi) {
-    const gpm_query& gq = queries[qi];
-    HostQuery hq;
-    hq.n = gq.n;
-    hq.adj.assign(gq.n, {});
-    hq.label.assign(gq.labels, gq.labels + gq.n);
-    for (uint32_t e = 0; e < gq.n_edges; ++e) {
-      const uint32_t a = gq.edges[2 * e], c = gq.edges[2 * e + 1];
-      hq.adj[a].push_back(c);
-      hq.adj[c].push_back(a);
-    }
-    for (auto& l : hq.adj) {
-      std::sort(l.begin(), l.end());
-      l.erase(std::unique(l.begin(), l.end()), l.end());
-    }
-    QPlan p;
-    if (!make_plan(g, hq, tun.policy, &p)) continue;  // a label the graph lacks: count 0
-    p.qid = (uint32_t)qi;
-    plans.push_back(p);
+HostQuery host_query(const gpm_query& gq) {
+  HostQuery hq;
+  hq.n = gq.n;
+  hq.adj.assign(gq.n, {});
+  hq.label.assign(gq.labels, gq.labels + gq.n);
+  for (uint32_t e = 0; e < gq.n_edges; ++e) {
+    const uint32_t a = gq.edges[2 * e], c = gq.edges[2 * e + 1];
+    hq.adj[a].push_back(c);
+    hq.adj[c].push_back(a);
   }
-  b->n_queries = n_queries;
-  b->nq = (uint32_t)plans.size();
-  const uint32_t nq = b->nq;
+  for (auto& l : hq.adj) {
+    std::sort(l.begin(), l.end());
+    l.erase(std::unique(l.begin(), l.end()), l.end());
+  }
+  return hq;
+}
+
+constexpr uint32_t kSplitLen = 128;      // depth-1 entries per part of a heavy root
+constexpr uint32_t kTrialTop = 16;       // heaviest roots every plan trial runs
+constexpr uint32_t kTrialSample = 1024;  // other roots a plan trial samples (about)
+
+uint32_t root_hash(uint32_t v) {
+  v ^= v >> 16;
+  v *= 0x7feb352dU;
+  v ^= v >> 15;
+  v *= 0x846ca68bU;
+  return v ^ (v >> 16);
+}
+
+// Lays out a batch of `plans` (plan.qid = index of its counter; n_counters
+// counters): device buffers, root candidates, chunks (roots per task) and
+// the parts of heavy roots.
+//
+// With rest_scale (a plan trial), plan i runs not all its root candidates
+// but two strata: its kTrialTop roots with the longest depth-1 slices
+// (counter 2i) and a hash-selected sample of about kTrialSample of the
+// others (counter 2i+1).  The batch then carries d_work, where the kernel
+// adds each counter's wave steps, and (*rest_scale)[i] scales the sample's
+// steps to all other roots of plan i.
+int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gpm_tuning& tun,
+                MatchBatch* b, std::vector<double>* rest_scale) {
+  const bool trial = rest_scale != nullptr;
+  const uint32_t nq0 = (uint32_t)plans.size();
   uint64_t root_total = 0;
   for (auto& p : plans) {
     p.root_off = (uint32_t)root_total;
     root_total += g->h_lvoff[p.label[0] + 1] - g->h_lvoff[p.label[0]];
   }
   if (root_total >= (1ull << 32)) return fail(GPM_EINVAL, "root candidate total too large");
+  const uint32_t plan_cap = trial ? 2 * nq0 : nq0;
+  const int n_slots = trial ? (int)(2 * nq0) : n_counters;
   auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
-  const size_t plan_bytes = align(std::max<size_t>(nq, 1) * sizeof(QPlan));
+  const size_t plan_bytes = align(std::max<size_t>(plan_cap, 1) * sizeof(QPlan));
   const size_t root_bytes = align(std::max<uint64_t>(root_total, 1) * 4);
-  const size_t count_bytes = align((size_t)(n_queries + 1) * 8);
-  const size_t small_bytes = align((size_t)(nq + 1 + 4) * 4);
+  const size_t count_bytes = align((size_t)(n_slots + 1) * 8);
+  const size_t work_bytes = trial ? count_bytes : 0;
+  const size_t small_bytes = align((size_t)(nq0 + 1 + 4) * 4);
   const size_t lvoff_bytes = align(g->h_lvoff.size() * 4);
-  const size_t total = plan_bytes + 2 * root_bytes + count_bytes + small_bytes + lvoff_bytes;
+  const size_t total =
+      plan_bytes + 2 * root_bytes + count_bytes + work_bytes + small_bytes + lvoff_bytes;
   if (hipMalloc(&b->mem, total) != hipSuccess) return fail(GPM_ENOMEM, "batch buffers (%zu B)", total);
-  char* p = static_cast<char*>(b->mem);
-  b->d_plans = reinterpret_cast<QPlan*>(p);
-  p += plan_bytes;
-  b->d_roots = reinterpret_cast<uint32_t*>(p);
-  p += root_bytes;
-  b->d_rlen = reinterpret_cast<uint32_t*>(p);
-  p += root_bytes;
-  b->d_counts = reinterpret_cast<unsigned long long*>(p);
-  p += count_bytes;
-  b->d_rootcnt = reinterpret_cast<unsigned int*>(p);
-  b->d_queue = b->d_rootcnt + nq + 1;
-  p += small_bytes;
-  uint32_t* d_lvoff = reinterpret_cast<uint32_t*>(p);
+  char* mp = static_cast<char*>(b->mem);
+  b->d_plans = reinterpret_cast<QPlan*>(mp);
+  mp += plan_bytes;
+  b->d_roots = reinterpret_cast<uint32_t*>(mp);
+  mp += root_bytes;
+  b->d_rlen = reinterpret_cast<uint32_t*>(mp);
+  mp += root_bytes;
+  b->d_counts = reinterpret_cast<unsigned long long*>(mp);
+  mp += count_bytes;
+  if (trial) {
+    b->d_work = reinterpret_cast<unsigned long long*>(mp);
+    mp += work_bytes;
+  }
+  b->d_rootcnt = reinterpret_cast<unsigned int*>(mp);
+  b->d_queue = b->d_rootcnt + nq0 + 1;
+  mp += small_bytes;
+  uint32_t* d_lvoff = reinterpret_cast<uint32_t*>(mp);
+  b->n_queries = n_slots;
+  b->nq = nq0;
   b->dg = DevGraph{g->off, g->ladj, g->loff, g->n_labels};
-  const int cus = device_cus(g);
-  b->grid = (unsigned)cus * 8;
-  if (nq == 0) return GPM_OK;
+  b->grid = (unsigned)device_cus(g) * 8;
+  if (nq0 == 0) return GPM_OK;
   int st;
   if ((st = h2d(g, d_lvoff, g->h_lvoff.data(), g->h_lvoff.size() * 4)) != GPM_OK) return st;
-  if ((st = h2d(g, b->d_plans, plans.data(), (size_t)nq * sizeof(QPlan))) != GPM_OK) return st;
-  hipLaunchKernelGGL(zero_kernel, dim3((nq + 64) / 64 + 1), dim3(64), 0, g->stream, b->d_counts,
-                     0, b->d_rootcnt, (int)nq);
+  if ((st = h2d(g, b->d_plans, plans.data(), (size_t)nq0 * sizeof(QPlan))) != GPM_OK) return st;
+  hipLaunchKernelGGL(zero_kernel, dim3((nq0 + 64) / 64 + 1), dim3(64), 0, g->stream, b->d_counts,
+                     0, b->d_rootcnt, (int)nq0);
   uint32_t maxlab = 0;
   for (auto& q : plans)
     maxlab = std::max(maxlab, g->h_lvoff[q.label[0] + 1] - g->h_lvoff[q.label[0]]);
   dim3 rgrid((unsigned)std::max<uint32_t>(1, std::min<uint32_t>((maxlab + kBlock - 1) / kBlock, 256)),
-             nq);
+             nq0);
   hipLaunchKernelGGL(roots_kernel, rgrid, dim3(kBlock), 0, g->stream, b->dg, g->lverts, d_lvoff,
                      b->d_plans, b->d_roots, b->d_rlen, b->d_rootcnt);
   GPM_HIP(hipGetLastError());
-  std::vector<unsigned int> rootcnt(nq);
-  GPM_HIP(hipMemcpyAsync(rootcnt.data(), b->d_rootcnt, nq * 4, hipMemcpyDeviceToHost, g->stream));
+  std::vector<unsigned int> rootcnt(nq0);
+  GPM_HIP(hipMemcpyAsync(rootcnt.data(), b->d_rootcnt, nq0 * 4, hipMemcpyDeviceToHost, g->stream));
   GPM_HIP(hipStreamSynchronize(g->stream));
-  for (uint32_t i = 0; i < nq; ++i) plans[i].n_roots = rootcnt[i];
+  for (uint32_t i = 0; i < nq0; ++i) plans[i].n_roots = rootcnt[i];
+  std::vector<uint32_t> rlen(root_total);
+  if (root_total) {
+    GPM_HIP(hipMemcpyAsync(rlen.data(), b->d_rlen, root_total * 4, hipMemcpyDeviceToHost,
+                           g->stream));
+    GPM_HIP(hipStreamSynchronize(g->stream));
+  }
+  if (trial) {
+    // Root lists come from atomics in arbitrary order; strata are chosen by
+    // (slice length, vertex id) and a hash of the id, so they do not depend
+    // on it.
+    std::vector<uint32_t> roots(root_total);
+    if (root_total) {
+      GPM_HIP(hipMemcpyAsync(roots.data(), b->d_roots, root_total * 4, hipMemcpyDeviceToHost,
+                             g->stream));
+      GPM_HIP(hipStreamSynchronize(g->stream));
+    }
+    std::vector<QPlan> strata;
+    std::vector<uint32_t> sroots, srlen;
+    rest_scale->assign(nq0, 0.0);
+    for (uint32_t i = 0; i < nq0; ++i) {
+      const QPlan& p = plans[i];
+      std::vector<uint32_t> idx(p.n_roots);
+      std::iota(idx.begin(), idx.end(), 0u);
+      const uint32_t n_top = std::min<uint32_t>(kTrialTop, p.n_roots);
+      auto heavier = [&](uint32_t x, uint32_t y) {
+        const uint32_t lx = rlen[p.root_off + x], ly = rlen[p.root_off + y];
+        return lx != ly ? lx > ly : roots[p.root_off + x] < roots[p.root_off + y];
+      };
+      std::partial_sort(idx.begin(), idx.begin() + n_top, idx.end(), heavier);
+      QPlan top = p, rest = p;
+      top.qid = 2 * i;
+      top.root_off = (uint32_t)sroots.size();
+      for (uint32_t k = 0; k < n_top; ++k) {
+        sroots.push_back(roots[p.root_off + idx[k]]);
+        srlen.push_back(rlen[p.root_off + idx[k]]);
+      }
+      top.n_roots = n_top;
+      rest.qid = 2 * i + 1;
+      rest.root_off = (uint32_t)sroots.size();
+      const uint32_t n_rest = p.n_roots - n_top;
+      const uint32_t stride = std::max<uint32_t>(1, (n_rest + kTrialSample - 1) / kTrialSample);
+      std::sort(idx.begin() + n_top, idx.end(),
+                [&](uint32_t x, uint32_t y) { return roots[p.root_off + x] < roots[p.root_off + y]; });
+      for (uint32_t k = n_top; k < p.n_roots; ++k) {
+        const uint32_t v = roots[p.root_off + idx[k]];
+        if (root_hash(v) % stride == 0) {
+          sroots.push_back(v);
+          srlen.push_back(rlen[p.root_off + idx[k]]);
+        }
+      }
+      if (n_rest && sroots.size() == rest.root_off) {  // nothing hashed in: take the lowest id
+        sroots.push_back(roots[p.root_off + idx[n_top]]);
+        srlen.push_back(rlen[p.root_off + idx[n_top]]);
+      }
+      rest.n_roots = (uint32_t)sroots.size() - rest.root_off;
+      (*rest_scale)[i] = rest.n_roots ? (double)n_rest / rest.n_roots : 0.0;
+      strata.push_back(top);
+      strata.push_back(rest);
+    }
+    if (!sroots.empty()) {  // a subset of the root lists: fits their buffers
+      if ((st = h2d(g, b->d_roots, sroots.data(), sroots.size() * 4)) != GPM_OK) return st;
+      if ((st = h2d(g, b->d_rlen, srlen.data(), srlen.size() * 4)) != GPM_OK) return st;
+    }
+    plans.swap(strata);
+    rlen.swap(srlen);
+  }
+  const uint32_t nq = (uint32_t)plans.size();
+  b->nq = nq;
   // Autotuner: estimated work per root of every query.
   std::vector<double> cost_per_root(nq, 1.0);
-  if (tun.autotune)
+  if (tun.autotune && !trial)
     for (uint32_t i = 0; i < nq; ++i) cost_per_root[i] = estimate_root_cost(g, plans[i]);
   // Chunks (roots per task) and layout, heaviest queries first when tuned.
+  // A tuned chunk gives a task about kTaskWork estimated entries, but no
+  // more roots than keep kTasksPerWave tasks per resident wave in the batch:
+  // light roots with large chunks would leave most waves idle.
   const double kTaskWork = 4096.0;  // estimated entries streamed per task
+  constexpr uint64_t kTasksPerWave = 2;
+  uint64_t batch_roots = 0;
+  for (const auto& q : plans) batch_roots += q.n_roots;
+  const uint64_t waves = (uint64_t)b->grid * kWavesPerBlock;
+  const double chunk_cap =
+      (double)std::max<uint64_t>(1, batch_roots / (kTasksPerWave * std::max<uint64_t>(waves, 1)));
   std::vector<uint32_t> perm(nq);
   std::iota(perm.begin(), perm.end(), 0u);
-  if (tun.autotune || tun.policy == GPM_SCHED_LEARNED)
+  if (!trial && (tun.autotune || tun.policy == GPM_SCHED_LEARNED))
     std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t c) {
       return cost_per_root[a] * plans[a].n_roots > cost_per_root[c] * plans[c].n_roots;
     });
@@ -779,10 +894,13 @@ int prepare_batch(gpm_graph* g, const gpm_query* queries, int n_queries, const g
   uint32_t t = 0;
   for (uint32_t i : perm) {
     QPlan q = plans[i];
-    if (tun.chunk)
+    if (trial)
+      q.chunk = 1;
+    else if (tun.chunk)
       q.chunk = tun.chunk;
     else if (tun.autotune)
-      q.chunk = (uint32_t)std::min(256.0, std::max(1.0, kTaskWork / cost_per_root[i]));
+      q.chunk = (uint32_t)std::min(std::min(256.0, chunk_cap),
+                                   std::max(1.0, kTaskWork / cost_per_root[i]));
     else
       q.chunk = 8;
     q.task_off = t;
@@ -793,27 +911,22 @@ int prepare_batch(gpm_graph* g, const gpm_query* queries, int n_queries, const g
   // of kSplitLen depth-1 entries each, so one hub's subtree is spread over
   // many waves; parts of the longest slices go first.  Queries whose tail
   // starts at depth 1 are counted per root without a DFS and are not split.
-  constexpr uint32_t kSplitLen = 128;
   b->split_len = kSplitLen;
   std::vector<HeavyTask> heavy;
-  if (root_total) {
-    std::vector<uint32_t> rlen(root_total);
-    GPM_HIP(hipMemcpyAsync(rlen.data(), b->d_rlen, root_total * 4, hipMemcpyDeviceToHost,
-                           g->stream));
-    GPM_HIP(hipStreamSynchronize(g->stream));
-    std::vector<uint32_t> hlen;
-    for (uint32_t li = 0; li < nq; ++li) {
-      const QPlan& q = laid[li];
-      if (q.tail <= 1) continue;
-      for (uint32_t r = 0; r < q.n_roots; ++r) {
-        const uint32_t len = rlen[q.root_off + r];
-        if (len <= kSplitLen) continue;
-        for (uint32_t lo = 0; lo < len; lo += kSplitLen) {
-          heavy.push_back({li, r, lo, std::min(len, lo + kSplitLen)});
-          hlen.push_back(len);
-        }
+  std::vector<uint32_t> hlen;
+  for (uint32_t li = 0; li < nq; ++li) {
+    const QPlan& q = laid[li];
+    if (q.tail <= 1) continue;
+    for (uint32_t r = 0; r < q.n_roots; ++r) {
+      const uint32_t len = rlen[q.root_off + r];
+      if (len <= kSplitLen) continue;
+      for (uint32_t lo = 0; lo < len; lo += kSplitLen) {
+        heavy.push_back({li, r, lo, std::min(len, lo + kSplitLen)});
+        hlen.push_back(len);
       }
     }
+  }
+  {
     std::vector<uint32_t> order(heavy.size());
     std::iota(order.begin(), order.end(), 0u);
     std::stable_sort(order.begin(), order.end(),
@@ -832,8 +945,72 @@ int prepare_batch(gpm_graph* g, const gpm_query* queries, int n_queries, const g
   }
   b->n_tasks = b->n_heavy + t;
   if ((st = h2d(g, b->d_plans, laid.data(), (size_t)nq * sizeof(QPlan))) != GPM_OK) return st;
-  if (tun.use_graph) return build_graph(b);
+  if (tun.use_graph && !trial) return build_graph(b);
   return GPM_OK;
+}
+
+// Plans every query.  Under the learned policy with the autotuner on, a
+// query whose cost-based order differs from the BFS order gets both plans
+// tried on a sample of their roots (one trial launch for the whole batch,
+// counting wave steps); the plan with fewer estimated steps over all its
+// roots is kept.  The estimate of the label statistics cannot see hubs, and
+// on power-law graphs a hub-heavy order can cost several times the BFS one.
+int plan_queries(gpm_graph* g, const gpm_query* queries, int n_queries, const gpm_tuning& tun,
+                 std::vector<QPlan>* out) {
+  std::vector<QPlan> trial_plans;
+  std::vector<size_t> at;  // index in *out of every tried query
+  for (int qi = 0; qi < n_queries; ++qi) {
+    const HostQuery hq = host_query(queries[qi]);
+    QPlan p;
+    if (!make_plan(g, hq, tun.policy, &p)) continue;  // a label the graph lacks: count 0
+    p.qid = (uint32_t)qi;
+    if (tun.policy == GPM_SCHED_LEARNED && tun.autotune) {
+      QPlan bfs;
+      if (make_plan(g, hq, GPM_SCHED_DEGREE, &bfs)) {
+        bfs.qid = p.qid;
+        if (std::memcmp(&bfs, &p, sizeof(QPlan)) != 0) {
+          trial_plans.push_back(p);
+          trial_plans.push_back(bfs);
+          at.push_back(out->size());
+        }
+      }
+    }
+    out->push_back(p);
+  }
+  if (trial_plans.empty()) return GPM_OK;
+  MatchBatch* t = new MatchBatch();
+  std::vector<double> scale;
+  int st = build_batch(g, trial_plans, 0, tun, t, &scale);
+  std::vector<unsigned long long> work((size_t)t->n_queries, 0ull);
+  if (st == GPM_OK && t->n_tasks) {
+    hipLaunchKernelGGL(zero_kernel, dim3((t->n_queries + 64) / 64 + 1), dim3(64), 0, g->stream,
+                       t->d_work, t->n_queries, nullptr, 0);
+    st = launch_batch(g, t);
+    if (st == GPM_OK && (hipMemcpyAsync(work.data(), t->d_work, work.size() * 8,
+                                        hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+                         hipStreamSynchronize(g->stream) != hipSuccess))
+      st = fail(GPM_EHIP, "plan trial");
+  }
+  free_batch(t);
+  if (st != GPM_OK) return st;
+  for (size_t j = 0; j < at.size(); ++j) {
+    double est[2];
+    for (int c = 0; c < 2; ++c) {
+      const size_t i = 2 * j + c;
+      est[c] = (double)work[2 * i] + (double)work[2 * i + 1] * scale[i];
+    }
+    if (est[1] < est[0]) (*out)[at[j]] = trial_plans[2 * j + 1];
+  }
+  return GPM_OK;
+}
+
+// Plans, filters roots, tunes and lays out a batch.
+int prepare_batch(gpm_graph* g, const gpm_query* queries, int n_queries, const gpm_tuning& tun,
+                  MatchBatch* b) {
+  std::vector<QPlan> plans;
+  int st = plan_queries(g, queries, n_queries, tun, &plans);
+  if (st != GPM_OK) return st;
+  return build_batch(g, std::move(plans), n_queries, tun, b, nullptr);
 }
 
 }  // namespace
