@@ -13,9 +13,9 @@
 //    embeddings smallest, and the start with the least expected total work
 //    wins.  The same estimate sizes the autotuner's chunks.  With the
 //    autotuner on, a query whose cost-based order differs from the BFS
-//    order gets both plans run on a sample of their roots (its heaviest
-//    roots plus a hash-selected share of the rest) in one trial launch that
-//    counts wave steps, and keeps the plan with fewer estimated steps.
+//    order gets both plans run on a hash-selected sample of their tasks
+//    (light roots and parts of heavy roots) in one trial launch that counts
+//    wave steps, and keeps the plan with fewer estimated steps.
 //  * Candidate filter at every depth: right label, enough degree and, per
 //    neighbour label, enough neighbours of that label (the NLF filter of
 //    myceci.cpp:120-169), read in O(1) per label from the label-partitioned
@@ -717,9 +717,8 @@ HostQuery host_query(const gpm_query& gq) {
   return hq;
 }
 
-constexpr uint32_t kSplitLen = 128;      // depth-1 entries per part of a heavy root
-constexpr uint32_t kTrialTop = 16;       // heaviest roots every plan trial runs
-constexpr uint32_t kTrialSample = 1024;  // other roots a plan trial samples (about)
+constexpr uint32_t kSplitLen = 128;    // depth-1 entries per part of a heavy root
+constexpr uint32_t kTrialUnits = 256;  // tasks a plan trial samples per plan (about)
 
 uint32_t root_hash(uint32_t v) {
   v ^= v >> 16;
@@ -729,19 +728,25 @@ uint32_t root_hash(uint32_t v) {
   return v ^ (v >> 16);
 }
 
+// Is the enumeration unit (root v, depth-1 entries from lo) in a trial
+// sample of one unit in `stride`?  Whole roots use lo = 0.
+bool in_sample(uint32_t v, uint32_t lo, uint32_t stride) {
+  return stride <= 1 || root_hash(v ^ root_hash(lo + 0x9e3779b9U)) % stride == 0;
+}
+
 // Lays out a batch of `plans` (plan.qid = index of its counter; n_counters
 // counters): device buffers, root candidates, chunks (roots per task) and
 // the parts of heavy roots.
 //
-// With rest_scale (a plan trial), plan i runs not all its root candidates
-// but two strata: its kTrialTop roots with the longest depth-1 slices
-// (counter 2i) and a hash-selected sample of about kTrialSample of the
-// others (counter 2i+1).  The batch then carries d_work, where the kernel
-// adds each counter's wave steps, and (*rest_scale)[i] scales the sample's
-// steps to all other roots of plan i.
+// With scale (a plan trial), plan i (counter i) runs a sample of its
+// enumeration units -- light roots, and the kSplitLen-entry parts of heavy
+// roots -- about kTrialUnits of them, picked by a hash of (vertex, part).
+// The batch then carries d_work, where the kernel adds each plan's wave
+// steps, and (*scale)[i] is units / sampled units of plan i.  Sampling parts
+// rather than roots keeps hubs represented without running them whole.
 int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gpm_tuning& tun,
-                MatchBatch* b, std::vector<double>* rest_scale) {
-  const bool trial = rest_scale != nullptr;
+                MatchBatch* b, std::vector<double>* scale) {
+  const bool trial = scale != nullptr;
   const uint32_t nq0 = (uint32_t)plans.size();
   uint64_t root_total = 0;
   for (auto& p : plans) {
@@ -749,8 +754,8 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
     root_total += g->h_lvoff[p.label[0] + 1] - g->h_lvoff[p.label[0]];
   }
   if (root_total >= (1ull << 32)) return fail(GPM_EINVAL, "root candidate total too large");
-  const uint32_t plan_cap = trial ? 2 * nq0 : nq0;
-  const int n_slots = trial ? (int)(2 * nq0) : n_counters;
+  const uint32_t plan_cap = nq0;
+  const int n_slots = trial ? (int)nq0 : n_counters;
   auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t plan_bytes = align(std::max<size_t>(plan_cap, 1) * sizeof(QPlan));
   const size_t root_bytes = align(std::max<uint64_t>(root_total, 1) * 4);
@@ -806,64 +811,53 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
                            g->stream));
     GPM_HIP(hipStreamSynchronize(g->stream));
   }
+  std::vector<uint32_t> stride(nq0, 1);  // trial: one sampled unit in stride[i]
+  std::vector<uint32_t> troots;          // trial: the sampled roots (host copy)
   if (trial) {
-    // Root lists come from atomics in arbitrary order; strata are chosen by
-    // (slice length, vertex id) and a hash of the id, so they do not depend
-    // on it.
+    // Root lists come from atomics in arbitrary order; the sample depends
+    // only on vertex ids and slice lengths, not on that order.
     std::vector<uint32_t> roots(root_total);
     if (root_total) {
       GPM_HIP(hipMemcpyAsync(roots.data(), b->d_roots, root_total * 4, hipMemcpyDeviceToHost,
                              g->stream));
       GPM_HIP(hipStreamSynchronize(g->stream));
     }
-    std::vector<QPlan> strata;
-    std::vector<uint32_t> sroots, srlen;
-    rest_scale->assign(nq0, 0.0);
+    std::vector<uint32_t>& sroots = troots;
+    std::vector<uint32_t> srlen;
+    scale->assign(nq0, 0.0);
     for (uint32_t i = 0; i < nq0; ++i) {
-      const QPlan& p = plans[i];
-      std::vector<uint32_t> idx(p.n_roots);
-      std::iota(idx.begin(), idx.end(), 0u);
-      const uint32_t n_top = std::min<uint32_t>(kTrialTop, p.n_roots);
-      auto heavier = [&](uint32_t x, uint32_t y) {
-        const uint32_t lx = rlen[p.root_off + x], ly = rlen[p.root_off + y];
-        return lx != ly ? lx > ly : roots[p.root_off + x] < roots[p.root_off + y];
+      QPlan& p = plans[i];
+      p.qid = i;
+      const bool split = p.tail > 1;
+      auto parts = [&](uint32_t len) {
+        return split && len > kSplitLen ? (len + kSplitLen - 1) / kSplitLen : 1u;
       };
-      std::partial_sort(idx.begin(), idx.begin() + n_top, idx.end(), heavier);
-      QPlan top = p, rest = p;
-      top.qid = 2 * i;
-      top.root_off = (uint32_t)sroots.size();
-      for (uint32_t k = 0; k < n_top; ++k) {
-        sroots.push_back(roots[p.root_off + idx[k]]);
-        srlen.push_back(rlen[p.root_off + idx[k]]);
-      }
-      top.n_roots = n_top;
-      rest.qid = 2 * i + 1;
-      rest.root_off = (uint32_t)sroots.size();
-      const uint32_t n_rest = p.n_roots - n_top;
-      const uint32_t stride = std::max<uint32_t>(1, (n_rest + kTrialSample - 1) / kTrialSample);
-      std::sort(idx.begin() + n_top, idx.end(),
-                [&](uint32_t x, uint32_t y) { return roots[p.root_off + x] < roots[p.root_off + y]; });
-      for (uint32_t k = n_top; k < p.n_roots; ++k) {
-        const uint32_t v = roots[p.root_off + idx[k]];
-        if (root_hash(v) % stride == 0) {
-          sroots.push_back(v);
-          srlen.push_back(rlen[p.root_off + idx[k]]);
+      uint64_t units = 0;
+      for (uint32_t r = 0; r < p.n_roots; ++r) units += parts(rlen[p.root_off + r]);
+      stride[i] = (uint32_t)std::max<uint64_t>(1, (units + kTrialUnits - 1) / kTrialUnits);
+      uint64_t picked = 0;
+      const uint32_t off0 = (uint32_t)sroots.size();
+      for (int pass = 0; pass < 2 && picked == 0 && units; ++pass) {
+        if (pass == 1) stride[i] = 1;  // nothing hashed in: take every unit
+        for (uint32_t r = 0; r < p.n_roots; ++r) {
+          const uint32_t v = roots[p.root_off + r], len = rlen[p.root_off + r];
+          uint32_t k = 0;
+          for (uint32_t u = 0; u < parts(len); ++u) k += in_sample(v, u * kSplitLen, stride[i]);
+          if (k) {
+            sroots.push_back(v);
+            srlen.push_back(len);
+            picked += k;
+          }
         }
       }
-      if (n_rest && sroots.size() == rest.root_off) {  // nothing hashed in: take the lowest id
-        sroots.push_back(roots[p.root_off + idx[n_top]]);
-        srlen.push_back(rlen[p.root_off + idx[n_top]]);
-      }
-      rest.n_roots = (uint32_t)sroots.size() - rest.root_off;
-      (*rest_scale)[i] = rest.n_roots ? (double)n_rest / rest.n_roots : 0.0;
-      strata.push_back(top);
-      strata.push_back(rest);
+      p.root_off = off0;
+      p.n_roots = (uint32_t)sroots.size() - off0;
+      (*scale)[i] = picked ? (double)units / (double)picked : 0.0;
     }
     if (!sroots.empty()) {  // a subset of the root lists: fits their buffers
       if ((st = h2d(g, b->d_roots, sroots.data(), sroots.size() * 4)) != GPM_OK) return st;
       if ((st = h2d(g, b->d_rlen, srlen.data(), srlen.size() * 4)) != GPM_OK) return st;
     }
-    plans.swap(strata);
     rlen.swap(srlen);
   }
   const uint32_t nq = (uint32_t)plans.size();
@@ -921,6 +915,7 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
       const uint32_t len = rlen[q.root_off + r];
       if (len <= kSplitLen) continue;
       for (uint32_t lo = 0; lo < len; lo += kSplitLen) {
+        if (trial && !in_sample(troots[q.root_off + r], lo, stride[q.qid])) continue;
         heavy.push_back({li, r, lo, std::min(len, lo + kSplitLen)});
         hlen.push_back(len);
       }
@@ -995,10 +990,7 @@ int plan_queries(gpm_graph* g, const gpm_query* queries, int n_queries, const gp
   if (st != GPM_OK) return st;
   for (size_t j = 0; j < at.size(); ++j) {
     double est[2];
-    for (int c = 0; c < 2; ++c) {
-      const size_t i = 2 * j + c;
-      est[c] = (double)work[2 * i] + (double)work[2 * i + 1] * scale[i];
-    }
+    for (int c = 0; c < 2; ++c) est[c] = (double)work[2 * j + c] * scale[2 * j + c];
     if (est[1] < est[0]) (*out)[at[j]] = trial_plans[2 * j + 1];
   }
   return GPM_OK;
