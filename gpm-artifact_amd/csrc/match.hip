@@ -12,10 +12,11 @@
 //    order grows by the vertex that keeps the expected number of partial
 //    embeddings smallest, and the start with the least expected total work
 //    wins.  The same estimate sizes the autotuner's chunks.  With the
-//    autotuner on, a query whose cost-based order differs from the BFS
-//    order gets both plans run on a hash-selected sample of their tasks
-//    (light roots and parts of heavy roots) in one trial launch that counts
-//    wave steps, and keeps the plan with fewer estimated steps.
+//    autotuner on, a query's candidate plans (the cost-based one, the BFS
+//    one, the cost-based orders from other starts) are run on a
+//    hash-selected sample of their tasks (light roots and parts of heavy
+//    roots) in one trial launch that counts wave steps, and the plan with
+//    the fewest estimated steps is kept.
 //  * Candidate filter at every depth: right label, enough degree and, per
 //    neighbour label, enough neighbours of that label (the NLF filter of
 //    myceci.cpp:120-169), read in O(1) per label from the label-partitioned
@@ -210,6 +211,26 @@ __device__ __forceinline__ bool passes_tail(const DevGraph& g, const QPlan& p, c
   return true;
 }
 
+// A trial task stops once its weighted steps pass this budget: a plan with
+// such a task is already far slower than one without, and the trial's
+// duration stays bounded whatever the candidates cost.
+constexpr uint64_t kTrialBudget = 1ull << 20;
+
+// Trial cost of one 64-entry window at depth d: the dependent loads one
+// lane makes -- the entry, its degree and NLF counts, and per probed
+// backward neighbour its slice bounds plus a binary search of that slice
+// (all neighbours when `full`, all but the streamed anchor otherwise).
+__device__ uint64_t window_cost(const DevGraph& g, const QPlan& p, const WaveState& s, uint32_t d,
+                                bool full) {
+  uint64_t c = 2 + p.nlf_n[d];
+  for (uint32_t bm = full ? p.bn[d] : (p.bn[d] & ~(1u << s.anc[d])); bm; bm &= bm - 1) {
+    uint32_t lo, len;
+    label_slice(g, s.emb[__builtin_ctz(bm)], p.label[d], &lo, &len);
+    c += 2 + (32 - __builtin_clz(len | 1u));
+  }
+  return c;
+}
+
 // Injective assignments of the tail depths given emb[0..tail-1]: the product
 // over equal-label groups of the inclusion-exclusion count over the group's
 // candidate sets A_1..A_k (k <= 3):
@@ -218,6 +239,7 @@ __device__ __forceinline__ bool passes_tail(const DevGraph& g, const QPlan& p, c
 // |A_B| for a subset B is counted while streaming the candidates of B's
 // first member (each x tested against the later members' full filters).
 // Arithmetic is modulo 2^64, exact when the true count fits.
+template <bool kTrial>
 __device__ uint64_t count_tail(const DevGraph& g, const QPlan& p, WaveState& s, uint64_t& steps) {
   const uint32_t lane = lane_id();
   uint64_t prod = 1;
@@ -228,8 +250,14 @@ __device__ uint64_t count_tail(const DevGraph& g, const QPlan& p, WaveState& s, 
       const uint32_t d = p.grp_dep[gi][m];
       init_depth(g, p, s, d);
       const uint32_t sl = uniform(s.sl[d]), sb = uniform(s.sb[d]);
+      uint64_t wcost = 1;
+      if (kTrial) {
+        wcost = window_cost(g, p, s, d, false);
+        for (uint32_t m2 = m + 1; m2 < k; ++m2) wcost += window_cost(g, p, s, p.grp_dep[gi][m2], true);
+      }
       for (uint32_t cur = 0; cur < sl; cur += kWave) {
-        ++steps;
+        if (kTrial && steps >= kTrialBudget) break;
+        steps += wcost;
         const uint32_t i = cur + lane;
         uint32_t mask = 0;
         if (i < sl) {
@@ -262,20 +290,21 @@ __device__ uint64_t count_tail(const DevGraph& g, const QPlan& p, WaveState& s, 
 // Counts the embeddings of plan p rooted at v0 whose depth-1 vertex is one
 // of the entries [lo1, hi1) of the depth-1 slice (all: 0, ~0u): a DFS over
 // depths 1..tail-1, each leaf of which adds the tail count.
+template <bool kTrial>
 __device__ uint64_t count_root(const DevGraph& g, const QPlan& p, WaveState& s, uint32_t v0,
                                uint32_t lo1, uint32_t hi1, uint64_t& steps) {
   const uint32_t lane = lane_id();
   const uint32_t n = p.n;
   if (n == 1) return 1;
   s.emb[0] = v0;
-  if (p.tail == 1) return count_tail(g, p, s, steps);
+  if (p.tail == 1) return count_tail<kTrial>(g, p, s, steps);
   uint64_t total = 0;
   init_depth(g, p, s, 1);
   s.cur[1] = lo1;
   s.sl[1] = min(s.sl[1], hi1);
   int d = 1;
   while (d >= 1) {
-    ++steps;
+    if (kTrial && steps >= kTrialBudget) break;
     const uint32_t dd = (uint32_t)d;
     if (s.mask[dd] == 0ull) {
       const uint32_t cur = s.cur[dd];
@@ -283,6 +312,7 @@ __device__ uint64_t count_root(const DevGraph& g, const QPlan& p, WaveState& s, 
         --d;
         continue;
       }
+      steps += kTrial ? window_cost(g, p, s, dd, false) : 1u;
       const uint32_t i = cur + lane;
       const bool ok = i < s.sl[dd] && passes(g, p, s, dd, g.ladj[s.sb[dd] + i]);
       s.mask[dd] = wave_ballot(ok);
@@ -295,9 +325,10 @@ __device__ uint64_t count_root(const DevGraph& g, const QPlan& p, WaveState& s, 
     s.mask[dd] = m & (m - 1);
     s.emb[dd] = uniform(g.ladj[s.sb[dd] + s.base[dd] + j]);
     if (dd + 1 == p.tail) {
-      total += count_tail(g, p, s, steps);
+      total += count_tail<kTrial>(g, p, s, steps);
     } else {
       init_depth(g, p, s, dd + 1);
+      steps += kTrial ? 2u * __builtin_popcount(p.bn[dd + 1]) : 1u;
       ++d;
     }
   }
@@ -325,8 +356,12 @@ struct HeavyTask {
 // Persistent enumeration: every wave pulls tasks from one queue until it is
 // drained.  Tasks [0, n_heavy) are parts of heavy roots (heaviest first);
 // the rest are chunks of a query's root list, which skip the heavy roots.
-// When `work` is given, the wave steps (ballot windows and descents) each
-// plan took are added to work[qid] as well.
+// Counted per task: steps = windows + descents.  A plan trial (kTrial)
+// weighs each window and descent by the dependent loads it makes, stops a
+// task past kTrialBudget (its counts are then partial and unused), and
+// adds, per plan, the total to work[3 qid], the largest task's to
+// work[3 qid + 1] and the number of stopped tasks to work[3 qid + 2].
+template <bool kTrial>
 __global__ __launch_bounds__(kBlock) void match_kernel(
     DevGraph g, const QPlan* __restrict__ plans, uint32_t nq, uint32_t n_tasks,
     const uint32_t* __restrict__ roots, const uint32_t* __restrict__ rlen,
@@ -348,8 +383,8 @@ __global__ __launch_bounds__(kBlock) void match_kernel(
       const HeavyTask h = heavy[t];
       const QPlan& p = plans[uniform(h.plan)];
       qid = p.qid;
-      c = count_root(g, p, s, uniform(roots[p.root_off + uniform(h.root)]), uniform(h.lo),
-                     uniform(h.hi), steps);
+      c = count_root<kTrial>(g, p, s, uniform(roots[p.root_off + uniform(h.root)]),
+                             uniform(h.lo), uniform(h.hi), steps);
     } else {
       const uint32_t pi = uniform(find_plan(plans, nq, t - n_heavy));
       const QPlan& p = plans[pi];
@@ -358,11 +393,16 @@ __global__ __launch_bounds__(kBlock) void match_kernel(
       const uint32_t r1 = min(r0 + p.chunk, p.n_roots);
       for (uint32_t r = r0; r < r1; ++r) {
         if (p.tail > 1 && uniform(rlen[p.root_off + r]) > split_len) continue;  // a heavy task
-        c += count_root(g, p, s, uniform(roots[p.root_off + r]), 0u, 0xFFFFFFFFu, steps);
+        c += count_root<kTrial>(g, p, s, uniform(roots[p.root_off + r]), 0u, 0xFFFFFFFFu,
+                                steps);
       }
     }
     if (lane == 0 && c) atomicAdd(&counts[qid], (unsigned long long)c);
-    if (work && lane == 0 && steps) atomicAdd(&work[qid], (unsigned long long)steps);
+    if (kTrial && lane == 0 && steps) {
+      atomicAdd(&work[3 * qid], (unsigned long long)steps);
+      atomicMax(&work[3 * qid + 1], (unsigned long long)steps);
+      if (steps >= kTrialBudget) atomicAdd(&work[3 * qid + 2], 1ull);
+    }
   }
 }
 
@@ -431,12 +471,57 @@ double estimate_root_cost(const gpm_graph* g, const QPlan& p) {
   return std::max(1.0, work);
 }
 
-// Builds the plan of one query; returns false when it cannot match at all
-// (a label the data graph does not have).
-bool make_plan(const gpm_graph* g, const HostQuery& q, int policy, QPlan* p) {
+// Cost-based order from start vertex s0: the order grows greedily by the
+// vertex that keeps the expected number of partial embeddings smallest.
+// *total = root candidates x expected work per root.  False when the query
+// is disconnected (checked by the caller).
+bool greedy_order(const gpm_graph* g, const HostQuery& q, uint32_t s0, std::vector<uint32_t>* out,
+                  double* total) {
   const uint32_t n = q.n;
-  for (uint32_t u = 0; u < n; ++u)
-    if (q.label[u] >= g->n_labels) return false;
+  const LabelStats st{g};
+  std::vector<uint32_t> ord{s0};
+  std::vector<char> in(n, 0);
+  in[s0] = 1;
+  uint32_t lab_of[kMaxQ];
+  std::vector<uint32_t> at(n, 0);
+  lab_of[0] = q.label[s0];
+  double nodes = 1.0, work = 1.0;
+  while (ord.size() < n) {
+    int bu = -1;
+    double bnodes = 0.0, bwork = 0.0;
+    int bconn = -1;
+    for (uint32_t u = 0; u < n; ++u) {
+      if (in[u]) continue;
+      uint32_t bn = 0;
+      for (uint32_t w : q.adj[u])
+        if (in[w]) bn |= 1u << at[w];
+      if (!bn) continue;
+      double nd = nodes;
+      const double wk = estimate_depth(st, lab_of, bn, q.label[u], &nd);
+      const int conn = __builtin_popcount(bn);
+      if (bu < 0 || nd < bnodes || (nd == bnodes && conn > bconn)) {
+        bu = (int)u;
+        bnodes = nd;
+        bwork = wk;
+        bconn = conn;
+      }
+    }
+    if (bu < 0) return false;
+    at[bu] = (uint32_t)ord.size();
+    lab_of[ord.size()] = q.label[bu];
+    in[bu] = 1;
+    ord.push_back((uint32_t)bu);
+    nodes = bnodes;
+    work += bwork;
+  }
+  *total = st.nv(q.label[s0]) * work;
+  *out = ord;
+  return true;
+}
+
+// CECI's start rule (least |V(label)| / degree) and the BFS order from it.
+std::vector<uint32_t> bfs_order(const gpm_graph* g, const HostQuery& q) {
+  const uint32_t n = q.n;
   auto lab_count = [&](uint32_t l) { return (double)(g->h_lvoff[l + 1] - g->h_lvoff[l]); };
   uint32_t start = 0;
   double best = 1e300;
@@ -450,71 +535,31 @@ bool make_plan(const gpm_graph* g, const HostQuery& q, int policy, QPlan* p) {
   }
   std::vector<uint32_t> order;
   std::vector<char> placed(n, 0);
-  if (policy == GPM_SCHED_LEARNED) {
-    // Cost-based order: from every start vertex, grow the order greedily by
-    // the vertex that keeps the expected number of partial embeddings
-    // smallest; keep the start whose whole order has the least expected work
-    // (root candidates x expected work per root).
-    const LabelStats st{g};
-    double best_total = 1e300;
-    for (uint32_t s0 = 0; s0 < n; ++s0) {
-      std::vector<uint32_t> ord{s0};
-      std::vector<char> in(n, 0);
-      in[s0] = 1;
-      uint32_t lab_of[kMaxQ];
-      std::vector<uint32_t> at(n, 0);
-      lab_of[0] = q.label[s0];
-      double nodes = 1.0, work = 1.0;
-      while (ord.size() < n) {
-        int bu = -1;
-        double bnodes = 0.0, bwork = 0.0;
-        int bconn = -1;
-        for (uint32_t u = 0; u < n; ++u) {
-          if (in[u]) continue;
-          uint32_t bn = 0;
-          for (uint32_t w : q.adj[u])
-            if (in[w]) bn |= 1u << at[w];
-          if (!bn) continue;
-          double nd = nodes;
-          const double wk = estimate_depth(st, lab_of, bn, q.label[u], &nd);
-          const int conn = __builtin_popcount(bn);
-          if (bu < 0 || nd < bnodes || (nd == bnodes && conn > bconn)) {
-            bu = (int)u;
-            bnodes = nd;
-            bwork = wk;
-            bconn = conn;
-          }
-        }
-        if (bu < 0) return false;  // disconnected (checked by the caller)
-        at[bu] = (uint32_t)ord.size();
-        lab_of[ord.size()] = q.label[bu];
-        in[bu] = 1;
-        ord.push_back((uint32_t)bu);
-        nodes = bnodes;
-        work += bwork;
+  std::queue<uint32_t> bq;
+  bq.push(start);
+  placed[start] = 1;
+  while (!bq.empty()) {
+    const uint32_t u = bq.front();
+    bq.pop();
+    order.push_back(u);
+    for (uint32_t w : q.adj[u])
+      if (!placed[w]) {
+        placed[w] = 1;
+        bq.push(w);
       }
-      const double total = st.nv(q.label[s0]) * work;
-      if (total < best_total) {
-        best_total = total;
-        order = ord;
-      }
-    }
-    for (uint32_t u : order) placed[u] = 1;
-  } else {
-    std::queue<uint32_t> bq;
-    bq.push(start);
-    placed[start] = 1;
-    while (!bq.empty()) {
-      const uint32_t u = bq.front();
-      bq.pop();
-      order.push_back(u);
-      for (uint32_t w : q.adj[u])
-        if (!placed[w]) {
-          placed[w] = 1;
-          bq.push(w);
-        }
-    }
   }
+  return order;
+}
+
+bool labels_known(const gpm_graph* g, const HostQuery& q) {
+  for (uint32_t u = 0; u < q.n; ++u)
+    if (q.label[u] >= g->n_labels) return false;
+  return true;
+}
+
+// Turns a matching order into a plan.
+void finish_plan(const HostQuery& q, std::vector<uint32_t> order, QPlan* p) {
+  const uint32_t n = q.n;
   // Move vertices whose neighbours all come earlier to the end (keeping the
   // root first), so the independent tail -- counted, not enumerated -- is as
   // long as possible.  Every moved vertex keeps all its backward neighbours.
@@ -581,7 +626,57 @@ bool make_plan(const gpm_graph* g, const HostQuery& q, int policy, QPlan* p) {
     if (gi == p->n_groups) p->grp_size[p->n_groups++] = 0;
     p->grp_dep[gi][p->grp_size[gi]++] = d;
   }
+}
+
+// Builds the plan of one query: BFS order (plain) or the cost-based order
+// from the start with the least expected total work (learned).  Returns
+// false when it cannot match at all (a label the data graph does not have).
+bool make_plan(const gpm_graph* g, const HostQuery& q, int policy, QPlan* p) {
+  if (!labels_known(g, q)) return false;
+  if (policy != GPM_SCHED_LEARNED) {
+    finish_plan(q, bfs_order(g, q), p);
+    return true;
+  }
+  std::vector<uint32_t> order, ord;
+  double best = 1e300, total = 0.0;
+  for (uint32_t s0 = 0; s0 < q.n; ++s0)
+    if (greedy_order(g, q, s0, &ord, &total) && total < best) {
+      best = total;
+      order = ord;
+    }
+  if (order.empty()) return false;
+  finish_plan(q, order, p);
   return true;
+}
+
+constexpr size_t kMaxCandidates = 6;  // plans a trial compares per query
+
+// Distinct plans a trial compares for one query: the learned plan, the BFS
+// plan, and the cost-based orders from the other start vertices in order of
+// estimated work, at most kMaxCandidates.
+std::vector<QPlan> plan_candidates(const gpm_graph* g, const HostQuery& q) {
+  std::vector<QPlan> out;
+  if (!labels_known(g, q)) return out;
+  auto add = [&](const std::vector<uint32_t>& order) {
+    if (out.size() >= kMaxCandidates) return;
+    QPlan p;
+    finish_plan(q, order, &p);
+    for (const QPlan& o : out)
+      if (std::memcmp(&o, &p, sizeof(QPlan)) == 0) return;
+    out.push_back(p);
+  };
+  std::vector<std::pair<double, std::vector<uint32_t>>> starts;
+  std::vector<uint32_t> ord;
+  double total = 0.0;
+  for (uint32_t s0 = 0; s0 < q.n; ++s0)
+    if (greedy_order(g, q, s0, &ord, &total)) starts.emplace_back(total, ord);
+  std::stable_sort(starts.begin(), starts.end(),
+                   [](const auto& x, const auto& y) { return x.first < y.first; });
+  if (starts.empty()) return out;
+  add(starts[0].second);
+  add(bfs_order(g, q));
+  for (size_t i = 1; i < starts.size(); ++i) add(starts[i].second);
+  return out;
 }
 
 bool connected(const HostQuery& q) {
@@ -663,7 +758,8 @@ int launch_batch(gpm_graph* g, MatchBatch* b) {
   }
   hipLaunchKernelGGL(zero_kernel, dim3((b->n_queries + 64) / 64 + 1), dim3(64), 0, g->stream,
                      b->d_counts, b->n_queries, b->d_queue, 4);
-  hipLaunchKernelGGL(match_kernel, dim3(b->grid), dim3(kBlock), 0, g->stream, b->dg,
+  hipLaunchKernelGGL(b->d_work ? match_kernel<true> : match_kernel<false>, dim3(b->grid),
+                     dim3(kBlock), 0, g->stream, b->dg,
                      b->d_plans, b->nq, b->n_tasks, b->d_roots, b->d_rlen, b->d_heavy,
                      b->n_heavy, b->split_len, b->d_queue, b->d_counts, b->d_work);
   GPM_HIP(hipGetLastError());
@@ -689,7 +785,7 @@ int build_graph(MatchBatch* b) {
                    &b->d_roots, &b->d_rlen,    &b->d_heavy, &b->n_heavy,
                    &b->split_len, &b->d_queue, &b->d_counts, &b->d_work};
   hipKernelNodeParams mp{};
-  mp.func = reinterpret_cast<void*>(&match_kernel);
+  mp.func = reinterpret_cast<void*>(&match_kernel<false>);
   mp.gridDim = dim3(b->grid);
   mp.blockDim = dim3(kBlock);
   mp.sharedMemBytes = 0;
@@ -718,7 +814,7 @@ HostQuery host_query(const gpm_query& gq) {
 }
 
 constexpr uint32_t kSplitLen = 128;    // depth-1 entries per part of a heavy root
-constexpr uint32_t kTrialUnits = 256;  // tasks a plan trial samples per plan (about)
+constexpr uint32_t kTrialUnits = 128;  // tasks a plan trial samples per plan (about)
 
 uint32_t root_hash(uint32_t v) {
   v ^= v >> 16;
@@ -740,10 +836,13 @@ bool in_sample(uint32_t v, uint32_t lo, uint32_t stride) {
 //
 // With scale (a plan trial), plan i (counter i) runs a sample of its
 // enumeration units -- light roots, and the kSplitLen-entry parts of heavy
-// roots -- about kTrialUnits of them, picked by a hash of (vertex, part).
-// The batch then carries d_work, where the kernel adds each plan's wave
-// steps, and (*scale)[i] is units / sampled units of plan i.  Sampling parts
-// rather than roots keeps hubs represented without running them whole.
+// roots -- about kTrialUnits of them, picked by a hash of (vertex, part),
+// plus the first part of its root with the longest depth-1 slice.  The
+// batch then carries d_work, where the kernel adds each plan's weighted
+// steps (work[3i]: total, work[3i+1]: largest task, work[3i+2]: tasks
+// stopped at the trial budget), and (*scale)[i] is units / sampled units of
+// plan i.  Sampling parts rather than roots keeps
+// hubs represented without running them whole.
 int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gpm_tuning& tun,
                 MatchBatch* b, std::vector<double>* scale) {
   const bool trial = scale != nullptr;
@@ -760,7 +859,7 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
   const size_t plan_bytes = align(std::max<size_t>(plan_cap, 1) * sizeof(QPlan));
   const size_t root_bytes = align(std::max<uint64_t>(root_total, 1) * 4);
   const size_t count_bytes = align((size_t)(n_slots + 1) * 8);
-  const size_t work_bytes = trial ? count_bytes : 0;
+  const size_t work_bytes = trial ? align((size_t)(3 * nq0 + 1) * 8) : 0;
   const size_t small_bytes = align((size_t)(nq0 + 1 + 4) * 4);
   const size_t lvoff_bytes = align(g->h_lvoff.size() * 4);
   const size_t total =
@@ -812,7 +911,11 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
     GPM_HIP(hipStreamSynchronize(g->stream));
   }
   std::vector<uint32_t> stride(nq0, 1);  // trial: one sampled unit in stride[i]
+  std::vector<uint32_t> vmax(nq0, 0);    // trial: root with the longest depth-1 slice
   std::vector<uint32_t> troots;          // trial: the sampled roots (host copy)
+  auto sampled = [&](uint32_t i, uint32_t v, uint32_t lo) {
+    return in_sample(v, lo, stride[i]) || (v == vmax[i] && lo == 0);
+  };
   if (trial) {
     // Root lists come from atomics in arbitrary order; the sample depends
     // only on vertex ids and slice lengths, not on that order.
@@ -833,7 +936,15 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
         return split && len > kSplitLen ? (len + kSplitLen - 1) / kSplitLen : 1u;
       };
       uint64_t units = 0;
-      for (uint32_t r = 0; r < p.n_roots; ++r) units += parts(rlen[p.root_off + r]);
+      uint32_t lmax = 0;
+      for (uint32_t r = 0; r < p.n_roots; ++r) {
+        const uint32_t len = rlen[p.root_off + r], v = roots[p.root_off + r];
+        units += parts(len);
+        if (r == 0 || len > lmax || (len == lmax && v < vmax[i])) {
+          lmax = len;
+          vmax[i] = v;
+        }
+      }
       stride[i] = (uint32_t)std::max<uint64_t>(1, (units + kTrialUnits - 1) / kTrialUnits);
       uint64_t picked = 0;
       const uint32_t off0 = (uint32_t)sroots.size();
@@ -842,7 +953,7 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
         for (uint32_t r = 0; r < p.n_roots; ++r) {
           const uint32_t v = roots[p.root_off + r], len = rlen[p.root_off + r];
           uint32_t k = 0;
-          for (uint32_t u = 0; u < parts(len); ++u) k += in_sample(v, u * kSplitLen, stride[i]);
+          for (uint32_t u = 0; u < parts(len); ++u) k += sampled(i, v, u * kSplitLen);
           if (k) {
             sroots.push_back(v);
             srlen.push_back(len);
@@ -915,7 +1026,7 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
       const uint32_t len = rlen[q.root_off + r];
       if (len <= kSplitLen) continue;
       for (uint32_t lo = 0; lo < len; lo += kSplitLen) {
-        if (trial && !in_sample(troots[q.root_off + r], lo, stride[q.qid])) continue;
+        if (trial && !sampled(q.qid, troots[q.root_off + r], lo)) continue;
         heavy.push_back({li, r, lo, std::min(len, lo + kSplitLen)});
         hlen.push_back(len);
       }
@@ -945,28 +1056,32 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
 }
 
 // Plans every query.  Under the learned policy with the autotuner on, a
-// query whose cost-based order differs from the BFS order gets both plans
-// tried on a sample of their roots (one trial launch for the whole batch,
-// counting wave steps); the plan with fewer estimated steps over all its
-// roots is kept.  The estimate of the label statistics cannot see hubs, and
-// on power-law graphs a hub-heavy order can cost several times the BFS one.
+// query with more than one candidate plan (plan_candidates: the cost-based
+// plan, the BFS plan, the cost-based orders from other starts) gets them all
+// tried on a sample of their tasks -- one trial launch for the whole batch,
+// counting wave steps -- and keeps the one with the fewest estimated steps.
+// The label-statistics estimate assumes uniform adjacency and cannot see
+// hubs: on power-law graphs the order it prefers can cost several times
+// another one, and which start is best changes with the graph.
 int plan_queries(gpm_graph* g, const gpm_query* queries, int n_queries, const gpm_tuning& tun,
                  std::vector<QPlan>* out) {
   std::vector<QPlan> trial_plans;
-  std::vector<size_t> at;  // index in *out of every tried query
+  struct Tried {
+    size_t at, first, count;  // index in *out; candidates in trial_plans
+  };
+  std::vector<Tried> tried;
   for (int qi = 0; qi < n_queries; ++qi) {
     const HostQuery hq = host_query(queries[qi]);
     QPlan p;
     if (!make_plan(g, hq, tun.policy, &p)) continue;  // a label the graph lacks: count 0
     p.qid = (uint32_t)qi;
     if (tun.policy == GPM_SCHED_LEARNED && tun.autotune) {
-      QPlan bfs;
-      if (make_plan(g, hq, GPM_SCHED_DEGREE, &bfs)) {
-        bfs.qid = p.qid;
-        if (std::memcmp(&bfs, &p, sizeof(QPlan)) != 0) {
-          trial_plans.push_back(p);
-          trial_plans.push_back(bfs);
-          at.push_back(out->size());
+      std::vector<QPlan> cand = plan_candidates(g, hq);
+      if (cand.size() > 1) {
+        tried.push_back({out->size(), trial_plans.size(), cand.size()});
+        for (QPlan& c : cand) {
+          c.qid = p.qid;
+          trial_plans.push_back(c);
         }
       }
     }
@@ -976,22 +1091,39 @@ int plan_queries(gpm_graph* g, const gpm_query* queries, int n_queries, const gp
   MatchBatch* t = new MatchBatch();
   std::vector<double> scale;
   int st = build_batch(g, trial_plans, 0, tun, t, &scale);
-  std::vector<unsigned long long> work((size_t)t->n_queries, 0ull);
+  std::vector<unsigned long long> work(3 * (size_t)t->n_queries, 0ull);
   if (st == GPM_OK && t->n_tasks) {
-    hipLaunchKernelGGL(zero_kernel, dim3((t->n_queries + 64) / 64 + 1), dim3(64), 0, g->stream,
-                       t->d_work, t->n_queries, nullptr, 0);
+    hipLaunchKernelGGL(zero_kernel, dim3((3 * t->n_queries + 64) / 64 + 1), dim3(64), 0,
+                       g->stream, t->d_work, 3 * t->n_queries, nullptr, 0);
     st = launch_batch(g, t);
     if (st == GPM_OK && (hipMemcpyAsync(work.data(), t->d_work, work.size() * 8,
                                         hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
                          hipStreamSynchronize(g->stream) != hipSuccess))
       st = fail(GPM_EHIP, "plan trial");
   }
+  // Estimated time of a plan in dependent-load rounds: its total work spread
+  // over the resident waves, or its largest task, whichever is longer.
+  // Plans with fewer (estimated) tasks stopped at the budget rank first --
+  // their other figures are truncated; ties keep the earlier, better
+  // estimated plan.
+  const double waves = (double)t->grid * kWavesPerBlock;
   free_batch(t);
   if (st != GPM_OK) return st;
-  for (size_t j = 0; j < at.size(); ++j) {
-    double est[2];
-    for (int c = 0; c < 2; ++c) est[c] = (double)work[2 * j + c] * scale[2 * j + c];
-    if (est[1] < est[0]) (*out)[at[j]] = trial_plans[2 * j + 1];
+  for (const Tried& tr : tried) {
+    size_t best = tr.first;
+    double best_est = 1e300;
+    double best_stopped = 1e300;
+    for (size_t i = tr.first; i < tr.first + tr.count; ++i) {
+      const double stopped = (double)work[3 * i + 2] * scale[i];
+      const double est =
+          std::max((double)work[3 * i] * scale[i] / waves, (double)work[3 * i + 1]);
+      if (stopped < best_stopped || (stopped == best_stopped && est < best_est)) {
+        best_stopped = stopped;
+        best_est = est;
+        best = i;
+      }
+    }
+    (*out)[tr.at] = trial_plans[best];
   }
   return GPM_OK;
 }
