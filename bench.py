@@ -223,7 +223,41 @@ def extra_match(scale):
             row["identical"] = row["plain"]["count"] == row["learned"]["count"]
             res[qname] = row
             print(f"[bench] {qname} done", file=sys.stderr, flush=True)
+        four = ("triangle", "4-cycle", "diamond", "three-star")
+        row = {"queries": list(four)}
+        for pol, pname in ((G.SCHED_DEGREE, "plain"), (G.SCHED_LEARNED, "learned")):
+            t0 = time.perf_counter()
+            c = dg.match_batch([named[q] for q in four], autotune=1, policy=pol)
+            first = (time.perf_counter() - t0) * 1e3
+            c2 = dg.match_batch([named[q] for q in four], autotune=1, policy=pol)
+            assert np.array_equal(c, c2)
+            assert [int(x) for x in c] == [res[q][pname]["count"] for q in four]
+            row[pname] = {"ms": dg.last_kernel_ms(), "first_call_ms": first}
+        row["learned_speedup"] = row["plain"]["ms"] / row["learned"]["ms"]
+        res["one_batch"] = row
     out["named"] = res
+    # Uneven batch: seeded random connected queries (random spanning tree plus
+    # each other pair with probability 0.5), 3-6 vertices, 8 labels.
+    rs = W.RANDOM_BATCH_SCALE
+    qs = W.random_queries(32, W.SEEDS["batch"], min_size=3, max_size=6, extra_edge_prob=0.5)
+    g, labels = W.labeled_graph(rs)
+    res = {"workload": f"32 seeded random connected queries (3-6 vertices, 8 labels) on "
+                       f"labeled R-MAT scale {rs}, one batch"}
+    with G.DeviceGraph(g, oriented=False, labels=labels, n_labels=W.N_LABELS) as dg:
+        counts = {}
+        for pname, pol in (("plain", G.SCHED_DEGREE), ("learned", G.SCHED_LEARNED)):
+            t0 = time.perf_counter()
+            c = dg.match_batch(qs, autotune=1, policy=pol)
+            first = (time.perf_counter() - t0) * 1e3
+            c2 = dg.match_batch(qs, autotune=1, policy=pol)
+            assert np.array_equal(c, c2)
+            counts[pname] = c
+            res[pname] = {"ms": dg.last_kernel_ms(), "first_call_ms": first}
+        res["identical"] = bool(np.array_equal(counts["plain"], counts["learned"]))
+        res["embeddings"] = int(counts["plain"].sum())
+        res["learned_speedup"] = res["plain"]["ms"] / res["learned"]["ms"]
+    out["random_batch"] = res
+    print("[bench] random batch done", file=sys.stderr, flush=True)
     return out
 
 
@@ -255,7 +289,7 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (to scale "
                          "profiler byte counters against a known traffic) and exit")
-    ap.add_argument("--match-scale", type=int, default=16)
+    ap.add_argument("--match-scale", type=int, default=18)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)

@@ -17,6 +17,7 @@ import numpy as np
 import gpm_amd as G
 
 N_LABELS = 8
+RANDOM_BATCH_SCALE = 14  # labeled R-MAT scale of the random-query batch
 SEEDS = {"tc": 20240601, "clique": 20240520, "labeled": 20240607, "queries": 7,
          "batch": 20240613}
 
