@@ -173,7 +173,7 @@ def extra_sched(scale):
     return out
 
 
-def extra_match(scale):
+def extra_match(scale, big_scale):
     """Labeled matching: the reference's 200 HPRD queries as one batch (exact
     against the reference's known answers), and the named queries on a
     labeled R-MAT graph."""
@@ -236,6 +236,23 @@ def extra_match(scale):
         row["learned_speedup"] = row["plain"]["ms"] / row["learned"]["ms"]
         res["one_batch"] = row
     out["named"] = res
+    # The same batch of four on the scale of the learned-scheduling workload.
+    g, labels = W.labeled_graph(big_scale)
+    res = {"workload": f"triangle, 4-cycle, diamond, three-star as one batch on labeled "
+                       f"R-MAT scale {big_scale} (8 labels)"}
+    with G.DeviceGraph(g, oriented=False, labels=labels, n_labels=W.N_LABELS) as dg:
+        counts = {}
+        for pol, pname in ((G.SCHED_DEGREE, "plain"), (G.SCHED_LEARNED, "learned")):
+            t0 = time.perf_counter()
+            c = dg.match_batch([named[q] for q in four], autotune=1, policy=pol)
+            first = (time.perf_counter() - t0) * 1e3
+            counts[pname] = [int(x) for x in c]
+            res[pname] = {"ms": dg.last_kernel_ms(), "first_call_ms": first}
+        res["counts"] = counts["plain"]
+        res["identical"] = counts["plain"] == counts["learned"]
+        res["learned_speedup"] = res["plain"]["ms"] / res["learned"]["ms"]
+    out["named_batch_large"] = res
+    print("[bench] large named batch done", file=sys.stderr, flush=True)
     # Uneven batch: seeded random connected queries (random spanning tree plus
     # each other pair with probability 0.5), 3-6 vertices, 8 labels.
     rs = W.RANDOM_BATCH_SCALE
@@ -405,7 +422,7 @@ def main():
         elif e == "sched":
             out["extra"]["sched"] = extra_sched(args.sched_scale)
         elif e == "match":
-            out["extra"]["match"] = extra_match(args.match_scale)
+            out["extra"]["match"] = extra_match(args.match_scale, args.sched_scale)
         print(f"[bench] extra {e} done in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
               flush=True)
     print(json.dumps(out), flush=True)
