@@ -213,7 +213,8 @@ void release(gpm_graph* g) {
   if (g->tc_exec) (void)hipGraphExecDestroy(g->tc_exec);
   if (g->tc_graph) (void)hipGraphDestroy(g->tc_graph);
   if (g->dag) gpm_graph_destroy(g->dag);
-  hipFree(g->rank);
+  (void)hipFree(g->rk_off);
+  (void)hipFree(g->rk_adj);
   free_tiers(g);
   hipFree(g->off);
   hipFree(g->adj);

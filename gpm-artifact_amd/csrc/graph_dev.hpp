@@ -56,9 +56,11 @@ struct gpm_graph {
   std::vector<std::pair<std::string, gpm::RootTiers>> tier_cache;
   // Cached algorithmic-byte totals per k (0 = not computed yet).
   uint64_t plan_bytes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  // Symmetric graphs: degree rank (position in (degree, id) order) and the
+  // Symmetric graphs: the graph relabelled by degree rank (position in
+  // (degree, id) order) with sorted lists (rk_off/rk_adj), and the
   // degree-oriented DAG, both built on first use by the motif counters.
-  uint32_t* rank = nullptr;
+  uint32_t* rk_off = nullptr;
+  uint32_t* rk_adj = nullptr;
   gpm_graph* dag = nullptr;
   // Last prepared matching batch (match.hip), replayed on identical calls.
   void* match_cache = nullptr;

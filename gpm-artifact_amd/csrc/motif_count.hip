@@ -4,9 +4,12 @@
 //   N_star    = sum_v C(d_v, 3)
 //   N_path    = sum_{edges uv} (d_u - 1)(d_v - 1) - 3T
 //   N_paw     = sum_v t_v (d_v - 2)            (tailed triangle)
+//             = 1/2 sum_{edges uv} c_uv (d_u + d_v - 4)
 //   N_cycle   = 4-cycles, counted per max-rank corner (below)
 //   N_diamond = sum_{edges e} C(c_e, 2)        (c_e = triangles on e)
 //   N_K4      = 4-cliques of the degree-oriented DAG
+// The edge sums take every undirected edge once (the DAG's edges) and
+// compute c_e on the fly.
 // and the induced counts by inclusion-exclusion over how many copies of
 // each shape a larger shape holds:
 //   I_K4 = N_K4
@@ -18,13 +21,17 @@
 //
 // 4-cycles: with rank = position in (degree, id) order, every 4-cycle has a
 // unique top corner u of maximal rank; its wedges u-v-w with v, w below u
-// land on the opposite corner w.  For each root u a block counts, for every
-// w, the wedges reaching it in a block-private counter array in HBM; each
+// land on the opposite corner w.  The walk runs on the graph relabelled by
+// rank with sorted lists, so "below u" is a prefix of each list (one binary
+// search, no rank lookups).  For each root u a block counts, for every w,
+// the wedges reaching it in a block-private counter array in HBM; each
 // increment adds the counter's previous value, which sums to
 // sum_w C(cnt_w, 2), and a second walk clears the counters.
 #include <algorithm>
 #include <numeric>
 #include <vector>
+
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "graph_dev.hpp"
 
@@ -34,16 +41,32 @@ namespace {
 
 constexpr int kBlock = 256;
 
-// c[e] = |N(u) & N(v)| for every directed edge e = (u, v); G lanes per edge.
+// Blocks of 256 threads for `threads` threads, at most 32 per CU (the
+// kernels stride over the rest).
+unsigned grid_of(const gpm_graph* g, uint64_t threads) {
+  const uint64_t want = (threads + 255) / 256;
+  return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)device_cus(g) * 32));
+}
+
+// Edge sums over the undirected edges {u, v} (each once: the edges of the
+// degree-oriented DAG), with c = |N(u) & N(v)| (triangles on the edge),
+// counted by G lanes per edge -- each element of the shorter list is
+// searched in the longer one -- and folded in at once:
+//   out[0] += 2c                      (sum: 6T)
+//   out[2] += c (d_u + d_v - 4)       (sum: 2 N_paw)
+//   out[3] += c (c - 1)               (sum: 2 N_diamond)
+//   out[4] += 2 (d_u - 1)(d_v - 1)    (sum: 2 x the path edge sum)
 template <int G>
-__global__ __launch_bounds__(kBlock) void edge_triangles_kernel(
+__global__ __launch_bounds__(kBlock) void edge_sums_kernel(
     const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
-    const uint32_t* __restrict__ src, uint64_t m, uint32_t* __restrict__ c) {
+    const uint32_t* __restrict__ esrc, const uint32_t* __restrict__ edst, uint64_t m,
+    unsigned long long* __restrict__ out) {
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t sub = threadIdx.x & (G - 1);
   const uint64_t ngrp = (uint64_t)gridDim.x * blockDim.x / G;
+  uint64_t s6t = 0, spaw = 0, sdia = 0, spath = 0;
   for (uint64_t e = tid / G; e < m; e += ngrp) {
-    const uint32_t u = src[e], v = adj[e];
+    const uint32_t u = esrc[e], v = edst[e];
     uint32_t ab = off[u], na = off[u + 1] - ab;
     uint32_t bb = off[v], nb = off[v + 1] - bb;
     if (na > nb) {
@@ -59,74 +82,73 @@ __global__ __launch_bounds__(kBlock) void edge_triangles_kernel(
     // Sum over the G lanes of this group (aligned, power of two).
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, kWave);
-    if (sub == 0) c[e] = cnt;
-  }
-}
-
-// Per-vertex sums (wave per vertex): out[0] += sum_e c_e (6T),
-// out[1] += sum_v C(d,3), out[2] += sum_v t_v (d_v - 2),
-// out[3] += sum_e C(c_e, 2) (directed: 2 N_diamond),
-// out[4] += sum_e (d_u - 1)(d_v - 1) (directed: 2 x the edge sum).
-__global__ __launch_bounds__(kBlock) void vertex_sums_kernel(
-    const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
-    const uint32_t* __restrict__ c, uint32_t n, unsigned long long* __restrict__ out) {
-  const uint32_t lane = lane_id();
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
-  uint64_t s6t = 0, sstar = 0, spaw = 0, sdia = 0, spath = 0;
-  for (uint32_t v = wave; v < n; v += nwaves) {
-    const uint32_t b = off[v], e = off[v + 1];
-    const uint64_t dv = e - b;
-    uint64_t t2 = 0, dia = 0, path = 0;
-    for (uint32_t i = b + lane; i < e; i += kWave) {
-      const uint64_t ce = c[i];
-      const uint32_t u = adj[i];
-      const uint64_t du = off[u + 1] - off[u];
-      t2 += ce;
-      dia += ce * (ce - (ce ? 1 : 0)) / 2;
-      path += (dv - 1) * (du - 1);
-    }
-    t2 = wave_sum_u64(t2);
-    dia = wave_sum_u64(dia);
-    path = wave_sum_u64(path);
-    if (lane == 0) {
-      s6t += t2;
-      sstar += dv >= 3 ? dv * (dv - 1) * (dv - 2) / 6 : 0;
-      spaw += dv >= 2 ? (t2 / 2) * (dv - 2) : 0;
-      sdia += dia;
-      spath += path;
+    if (sub == 0) {
+      const uint64_t c = cnt, da = na, db = nb;
+      s6t += 2 * c;
+      spaw += c ? c * (da + db - 4) : 0;
+      sdia += c * (c - (c ? 1 : 0));
+      spath += 2 * (da - 1) * (db - 1);
     }
   }
-  if (lane == 0) {
+  s6t = wave_sum_u64(s6t);
+  spaw = wave_sum_u64(spaw);
+  sdia = wave_sum_u64(sdia);
+  spath = wave_sum_u64(spath);
+  if (lane_id() == 0) {
     atomicAdd(&out[0], (unsigned long long)s6t);
-    atomicAdd(&out[1], (unsigned long long)sstar);
     atomicAdd(&out[2], (unsigned long long)spaw);
     atomicAdd(&out[3], (unsigned long long)sdia);
     atomicAdd(&out[4], (unsigned long long)spath);
   }
 }
 
-// Block per root u (dynamic queue).  PASS 0 counts (atomic increments whose
-// previous values sum to sum_w C(cnt_w, 2)); PASS 1 clears the counters.
+// out[1] += sum_v C(d_v, 3).
+__global__ __launch_bounds__(kBlock) void star_sum_kernel(const uint32_t* __restrict__ off,
+                                                          uint32_t n,
+                                                          unsigned long long* __restrict__ out) {
+  uint64_t acc = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const uint64_t d = off[v + 1] - off[v];
+    acc += d >= 3 ? d * (d - 1) * (d - 2) / 6 : 0;
+  }
+  acc = wave_sum_u64(acc);
+  if (lane_id() == 0 && acc) atomicAdd(&out[1], (unsigned long long)acc);
+}
+
+// Entries of the sorted list a[0..n) below key.
+__device__ __forceinline__ uint32_t count_below(const uint32_t* __restrict__ a, uint32_t n,
+                                                uint32_t key) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// 4-cycle walk of root u over the rank-relabelled graph (ids = rank, lists
+// sorted): the wedges u - v - x with v, x < u are the prefixes below u of
+// N(u) and of every such N(v).  PASS 0 counts (atomic increments whose
+// previous values sum to sum_x C(cnt_x, 2)); PASS 1 clears the counters.
 __device__ __forceinline__ void cycle_walk(const uint32_t* __restrict__ off,
-                                           const uint32_t* __restrict__ adj,
-                                           const uint32_t* __restrict__ rank, uint32_t u,
+                                           const uint32_t* __restrict__ adj, uint32_t u,
                                            uint32_t* __restrict__ cnt, bool clear,
                                            uint64_t* acc, uint32_t* s_pre, uint32_t* s_beg,
                                            uint32_t* s_wsum) {
-  const uint32_t ru = rank[u];
-  const uint32_t b = uniform(off[u]), d = uniform(off[u + 1]) - b;
-  for (uint32_t c0 = 0; c0 < d; c0 += kBlock) {
+  const uint32_t b = uniform(off[u]);
+  const uint32_t dl = uniform(count_below(adj + b, uniform(off[u + 1]) - b, u));
+  for (uint32_t c0 = 0; c0 < dl; c0 += kBlock) {
     const uint32_t i = c0 + threadIdx.x;
     uint32_t len = 0, vb = 0;
-    if (i < d) {
+    if (i < dl) {
       const uint32_t v = adj[b + i];
-      if (rank[v] < ru) {
-        vb = off[v];
-        len = off[v + 1] - vb;
-      }
+      vb = off[v];
+      len = count_below(adj + vb, off[v + 1] - vb, u);
     }
-    // Block scan of the qualifying list lengths.
+    // Block scan of the prefix lengths.
     const uint32_t lane = lane_id(), w = threadIdx.x / kWave;
     const uint32_t incl0 = wave_incl_scan(len);
     if (lane == kWave - 1) s_wsum[w] = incl0;
@@ -141,21 +163,20 @@ __device__ __forceinline__ void cycle_walk(const uint32_t* __restrict__ off,
       const uint32_t j = upper_index(s_pre, kBlock, f);
       const uint32_t excl = j ? s_pre[j - 1] : 0u;
       const uint32_t x = adj[s_beg[j] + (f - excl)];
-      if (x != u && rank[x] < ru) {
-        if (clear)
-          cnt[x] = 0u;
-        else
-          *acc += atomicAdd(&cnt[x], 1u);
-      }
+      if (clear)
+        cnt[x] = 0u;
+      else
+        *acc += atomicAdd(&cnt[x], 1u);
     }
     __syncthreads();
   }
 }
 
+// Block per root, highest rank first (dynamic queue).
 __global__ __launch_bounds__(kBlock) void cycles_kernel(
-    const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
-    const uint32_t* __restrict__ rank, uint32_t n, unsigned int* __restrict__ queue,
-    uint32_t* __restrict__ counters, uint64_t stride, unsigned long long* __restrict__ out) {
+    const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj, uint32_t n,
+    unsigned int* __restrict__ queue, uint32_t* __restrict__ counters, uint64_t stride,
+    unsigned long long* __restrict__ out) {
   __shared__ uint32_t s_pre[kBlock];
   __shared__ uint32_t s_beg[kBlock];
   __shared__ uint32_t s_wsum[kBlock / kWave];
@@ -165,15 +186,16 @@ __global__ __launch_bounds__(kBlock) void cycles_kernel(
   while (true) {
     if (threadIdx.x == 0) s_root = atomicAdd(queue, 1u);
     __syncthreads();
-    const uint32_t u = uniform(s_root);
+    const uint32_t q = uniform(s_root);
     __syncthreads();
-    if (u >= n) break;
-    cycle_walk(off, adj, rank, u, cnt, false, &acc, s_pre, s_beg, s_wsum);
+    if (q >= n) break;
+    const uint32_t u = n - 1 - q;
+    cycle_walk(off, adj, u, cnt, false, &acc, s_pre, s_beg, s_wsum);
     // Counter updates of this root must be complete before the clearing
-    // walk reads nothing but writes zeros; the fence orders them.
+    // walk writes zeros; the fence orders them.
     __threadfence();
     __syncthreads();
-    cycle_walk(off, adj, rank, u, cnt, true, &acc, s_pre, s_beg, s_wsum);
+    cycle_walk(off, adj, u, cnt, true, &acc, s_pre, s_beg, s_wsum);
     __threadfence();
     __syncthreads();
   }
@@ -182,23 +204,93 @@ __global__ __launch_bounds__(kBlock) void cycles_kernel(
 }
 
 __global__ void zero_u32_kernel(unsigned int* p, uint64_t count) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) p[i] = 0u;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
 }
 
-int ensure_rank(gpm_graph* g) {
-  if (g->rank) return GPM_OK;
-  std::vector<uint32_t> order(g->n), rank(g->n);
+// keys[e] = rank(u) << 32 | rank(x) for every adjacency entry e = (u, x).
+__global__ __launch_bounds__(kBlock) void rank_keys_kernel(const uint32_t* __restrict__ off,
+                                                           const uint32_t* __restrict__ adj,
+                                                           const uint32_t* __restrict__ rank,
+                                                           uint32_t n, uint64_t* __restrict__ keys) {
+  const uint32_t lane = lane_id();
+  const uint32_t nw = gridDim.x * blockDim.x / kWave;
+  for (uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; v < n; v += nw) {
+    const uint64_t hi = (uint64_t)rank[v] << 32;
+    const uint32_t b = off[v], e = off[v + 1];
+    for (uint32_t i = b + lane; i < e; i += kWave) keys[i] = hi | rank[adj[i]];
+  }
+}
+
+__global__ void low_words_kernel(const uint64_t* __restrict__ keys, uint64_t m,
+                                 uint32_t* __restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = (uint32_t)keys[i];
+}
+
+// The graph relabelled by degree rank (position in (degree, id) order),
+// every list sorted: rows of (rank(u), rank(x)) keys sorted as 64-bit
+// integers, the row offsets from the degrees in rank order.
+int ensure_rank_csr(gpm_graph* g) {
+  if (g->rk_off) return GPM_OK;
+  const uint32_t n = g->n;
+  const uint64_t m = g->m;
+  std::vector<uint32_t> order(n), rank(n), rk_off(n + 1, 0u);
   std::iota(order.begin(), order.end(), 0u);
   auto deg = [&](uint32_t v) { return g->h_off[v + 1] - g->h_off[v]; };
   std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
     return deg(a) < deg(b) || (deg(a) == deg(b) && a < b);
   });
-  for (uint32_t i = 0; i < g->n; ++i) rank[order[i]] = i;
-  if (hipMalloc(&g->rank, (size_t)std::max(g->n, 1u) * 4) != hipSuccess)
-    return fail(GPM_ENOMEM, "rank array");
-  int st = h2d(g, g->rank, rank.data(), (size_t)g->n * 4);
-  if (st != GPM_OK) return st;
+  for (uint32_t i = 0; i < n; ++i) {
+    rank[order[i]] = i;
+    rk_off[i + 1] = rk_off[i] + deg(order[i]);
+  }
+  uint32_t* d_rank = nullptr;
+  uint64_t* keys = nullptr;
+  void* temp = nullptr;
+  size_t temp_bytes = 0;
+  unsigned bits = 32;
+  while (bits < 64 && (1ull << (bits - 32)) < (uint64_t)std::max(n, 1u)) ++bits;
+  int st = GPM_OK;
+  auto cleanup = [&]() {
+    (void)hipFree(d_rank);
+    (void)hipFree(keys);
+    (void)hipFree(temp);
+  };
+  auto fail_all = [&](int code) {  // leaves no half-built graph on the handle
+    cleanup();
+    (void)hipFree(g->rk_off);
+    (void)hipFree(g->rk_adj);
+    g->rk_off = g->rk_adj = nullptr;
+    return code;
+  };
+  if (hipMalloc(&g->rk_off, (size_t)(n + 1) * 4) != hipSuccess ||
+      hipMalloc(&g->rk_adj, std::max<uint64_t>(m, 1) * 4) != hipSuccess ||
+      hipMalloc(&d_rank, (size_t)std::max(n, 1u) * 4) != hipSuccess ||
+      hipMalloc(&keys, std::max<uint64_t>(m, 1) * 16) != hipSuccess)
+    return fail_all(
+        fail(GPM_ENOMEM, "rank-relabelled graph (%llu entries)", (unsigned long long)m));
+  if ((st = h2d(g, g->rk_off, rk_off.data(), (size_t)(n + 1) * 4)) != GPM_OK ||
+      (st = h2d(g, d_rank, rank.data(), (size_t)n * 4)) != GPM_OK)
+    return fail_all(st);
+  if (m) {
+    uint64_t* sorted = keys + m;
+    hipLaunchKernelGGL(rank_keys_kernel, dim3(grid_of(g, (uint64_t)n * kWave)),
+                       dim3(kBlock), 0, g->stream, g->off, g->adj, d_rank, n, keys);
+    if (rocprim::radix_sort_keys(nullptr, temp_bytes, keys, sorted, m, 0u, bits, g->stream) !=
+            hipSuccess ||
+        hipMalloc(&temp, std::max<size_t>(temp_bytes, 1)) != hipSuccess ||
+        rocprim::radix_sort_keys(temp, temp_bytes, keys, sorted, m, 0u, bits, g->stream) !=
+            hipSuccess)
+      return fail_all(fail(GPM_EHIP, "sorting the rank-relabelled graph"));
+    hipLaunchKernelGGL(low_words_kernel, dim3(grid_of(g, m)), dim3(256), 0, g->stream,
+                       sorted, m, g->rk_adj);
+  }
+  if (hipStreamSynchronize(g->stream) != hipSuccess)
+    return fail_all(fail(GPM_EHIP, "building the rank-relabelled graph"));
+  cleanup();
   return GPM_OK;
 }
 
@@ -208,40 +300,36 @@ namespace gpm {
 
 int motif4_by_identities(gpm_graph* g, uint64_t out[6]) {
   int st;
-  if ((st = ensure_src(g)) != GPM_OK) return st;
-  if ((st = ensure_rank(g)) != GPM_OK) return st;
+  if ((st = ensure_rank_csr(g)) != GPM_OK) return st;
   if (!g->dag && (st = gpm_graph_orient(g, &g->dag)) != GPM_OK) return st;
+  if ((st = ensure_src(g->dag)) != GPM_OK) return st;
   const int cus = device_cus(g);
-  // Scratch: per-edge triangle counts, then block-private 4-cycle counters.
-  const uint64_t tri_bytes = std::max<uint64_t>(g->m, 1) * 4;
+  // Scratch: block-private 4-cycle counters (one per vertex), at most 4 GiB.
   const uint64_t stride = std::max<uint32_t>(g->n, 1);
   unsigned blocks_c = (unsigned)std::min<uint64_t>((uint64_t)cus * 2, std::max(g->n, 1u));
   while (blocks_c > 16 && (uint64_t)blocks_c * stride * 4 > (4ull << 30)) blocks_c /= 2;
   const uint64_t cyc_bytes = (uint64_t)blocks_c * stride * 4;
-  if ((st = ensure_scratch(g, tri_bytes + cyc_bytes + 256)) != GPM_OK) return st;
-  uint32_t* tri = static_cast<uint32_t*>(g->scratch);
-  uint32_t* ccnt = reinterpret_cast<uint32_t*>(static_cast<char*>(g->scratch) + tri_bytes);
+  if ((st = ensure_scratch(g, cyc_bytes + 256)) != GPM_OK) return st;
+  uint32_t* ccnt = static_cast<uint32_t*>(g->scratch);
   if ((st = timing_begin(g)) != GPM_OK) return st;
   if ((st = zero_counters(g, 16)) != GPM_OK) return st;
   unsigned int* queue = reinterpret_cast<unsigned int*>(g->counters + 8);
   const uint64_t zero_n = cyc_bytes / 4;
-  hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)((zero_n + 255) / 256)), dim3(256), 0,
-                     g->stream, ccnt, zero_n);
-  const unsigned eblocks =
-      (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((g->m * 8 + kBlock - 1) / kBlock,
-                                                         (uint64_t)cus * 32));
-  hipLaunchKernelGGL(edge_triangles_kernel<8>, dim3(eblocks), dim3(kBlock), 0, g->stream, g->off,
-                     g->adj, g->src, g->m, tri);
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_of(g, zero_n)), dim3(256), 0, g->stream, ccnt,
+                     zero_n);
+  const uint64_t me = g->dag->m;
+  if (me) {
+    hipLaunchKernelGGL(edge_sums_kernel<8>, dim3(grid_of(g, me * 8)), dim3(kBlock), 0, g->stream,
+                       g->off, g->adj, g->dag->src, g->dag->adj, me, g->counters);
+    GPM_HIP(hipGetLastError());
+  }
+  timing_mark(g, "motif_edge_sums");
+  hipLaunchKernelGGL(star_sum_kernel, dim3(grid_of(g, g->n)), dim3(kBlock), 0, g->stream, g->off,
+                     g->n, g->counters);
   GPM_HIP(hipGetLastError());
-  timing_mark(g, "motif_edge_triangles");
-  const unsigned vblocks = (unsigned)std::max<uint64_t>(
-      1, std::min<uint64_t>(((uint64_t)g->n * kWave + kBlock - 1) / kBlock, (uint64_t)cus * 16));
-  hipLaunchKernelGGL(vertex_sums_kernel, dim3(vblocks), dim3(kBlock), 0, g->stream, g->off,
-                     g->adj, tri, g->n, g->counters);
-  GPM_HIP(hipGetLastError());
-  timing_mark(g, "motif_vertex_sums");
-  hipLaunchKernelGGL(cycles_kernel, dim3(blocks_c), dim3(kBlock), 0, g->stream, g->off, g->adj,
-                     g->rank, g->n, queue, ccnt, stride, g->counters + 5);
+  timing_mark(g, "motif_star_sum");
+  hipLaunchKernelGGL(cycles_kernel, dim3(blocks_c), dim3(kBlock), 0, g->stream, g->rk_off,
+                     g->rk_adj, g->n, queue, ccnt, stride, g->counters + 5);
   GPM_HIP(hipGetLastError());
   timing_mark(g, "motif_cycles");
   if ((st = timing_end(g)) != GPM_OK) return st;
@@ -254,7 +342,7 @@ int motif4_by_identities(gpm_graph* g, uint64_t out[6]) {
   g->last_ms = ms_main + g->dag->last_ms;
   const uint64_t T = h[0] / 6;
   const uint64_t n_star = h[1];
-  const uint64_t n_paw = h[2];
+  const uint64_t n_paw = h[2] / 2;
   const uint64_t n_dia = h[3] / 2;
   const uint64_t n_path = h[4] / 2 - 3 * T;
   const uint64_t n_cyc = h[5];
