@@ -32,7 +32,10 @@ std::string key_of(const TierRequest& r, uint32_t n) {
 
 int upload(gpm_graph* g, const std::vector<uint32_t>& v, uint32_t** d) {
   if (v.empty()) return GPM_OK;
-  if (hipMalloc(d, v.size() * 4) != hipSuccess) return fail(GPM_ENOMEM, "tier list");
+  if (hipMalloc(d, v.size() * 4) != hipSuccess) {
+    *d = nullptr;
+    return fail(GPM_ENOMEM, "tier list");
+  }
   return h2d(g, *d, v.data(), v.size() * 4);
 }
 
@@ -162,16 +165,22 @@ int get_tiers(gpm_graph* g, const TierRequest& req, const RootTiers** out) {
     for (auto* tier : {&wave, &block, &block2, &big})
       for (uint32_t& v : *tier) v = req.rank[v];
   int st;
-  if ((st = upload(g, wave, &t.d_wave)) != GPM_OK) return st;
-  if ((st = upload(g, block, &t.d_block)) != GPM_OK) return st;
-  if ((st = upload(g, block2, &t.d_block2)) != GPM_OK) return st;
-  if ((st = upload(g, big, &t.d_big)) != GPM_OK) return st;
-  if ((st = upload(g, block_parts, &t.d_block_parts)) != GPM_OK) return st;
-  if ((st = upload(g, block2_parts, &t.d_block2_parts)) != GPM_OK) return st;
-  if ((st = upload(g, big_parts, &t.d_big_parts)) != GPM_OK) return st;
+  if ((st = upload(g, wave, &t.d_wave)) != GPM_OK ||
+      (st = upload(g, block, &t.d_block)) != GPM_OK ||
+      (st = upload(g, block2, &t.d_block2)) != GPM_OK ||
+      (st = upload(g, big, &t.d_big)) != GPM_OK ||
+      (st = upload(g, block_parts, &t.d_block_parts)) != GPM_OK ||
+      (st = upload(g, block2_parts, &t.d_block2_parts)) != GPM_OK ||
+      (st = upload(g, big_parts, &t.d_big_parts)) != GPM_OK) {
+    release(t);  // frees whatever the failed upload sequence allocated
+    return st;
+  }
   t.n_split = n_split;
-  if (hipMalloc(&t.queues, 4 * sizeof(unsigned int)) != hipSuccess)
+  if (hipMalloc(&t.queues, 4 * sizeof(unsigned int)) != hipSuccess) {
+    t.queues = nullptr;
+    release(t);
     return fail(GPM_ENOMEM, "work queues");
+  }
   g->tier_cache.emplace_back(key, t);
   *out = &g->tier_cache.back().second;
   // Keep the cache small: learned requests with changing costs would grow it.
