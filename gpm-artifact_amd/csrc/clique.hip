@@ -914,7 +914,10 @@ int gpm_clique_root_work(gpm_graph* g, int k, uint64_t* work) {
   if (st != GPM_OK) return st;
   unsigned long long* d_work = nullptr;
   GPM_HIP(hipMalloc(&d_work, (size_t)std::max(g->n, 1u) * 8));
-  GPM_HIP(hipMemsetAsync(d_work, 0, (size_t)std::max(g->n, 1u) * 8, g->stream));
+  if (hipMemsetAsync(d_work, 0, (size_t)std::max(g->n, 1u) * 8, g->stream) != hipSuccess) {
+    (void)hipFree(d_work);
+    return fail(GPM_EHIP, "clearing the work log failed");
+  }
   st = launch_k<kWork>(g, k, t, g->counters, d_work);
   // The kernels log by topological id; report by vertex id.
   std::vector<uint64_t> by_rank(g->n);
