@@ -26,7 +26,8 @@
 // search, no rank lookups).  For each root u a block counts, for every w,
 // the wedges reaching it in a block-private counter array in HBM; each
 // increment adds the counter's previous value, which sums to
-// sum_w C(cnt_w, 2), and a second walk clears the counters.
+// sum_w C(cnt_w, 2).  The first increment of a counter also appends w to a
+// block-private list, and only the listed counters are cleared afterwards.
 #include <algorithm>
 #include <numeric>
 #include <vector>
@@ -131,11 +132,13 @@ __device__ __forceinline__ uint32_t count_below(const uint32_t* __restrict__ a, 
 
 // 4-cycle walk of root u over the rank-relabelled graph (ids = rank, lists
 // sorted): the wedges u - v - x with v, x < u are the prefixes below u of
-// N(u) and of every such N(v).  PASS 0 counts (atomic increments whose
-// previous values sum to sum_x C(cnt_x, 2)); PASS 1 clears the counters.
+// N(u) and of every such N(v).  Atomic increments whose previous values sum
+// to sum_x C(cnt_x, 2); every x seen first is appended to `touched`
+// (*s_nt entries).
 __device__ __forceinline__ void cycle_walk(const uint32_t* __restrict__ off,
                                            const uint32_t* __restrict__ adj, uint32_t u,
-                                           uint32_t* __restrict__ cnt, bool clear,
+                                           uint32_t* __restrict__ cnt,
+                                           uint32_t* __restrict__ touched, uint32_t* s_nt,
                                            uint64_t* acc, uint32_t* s_pre, uint32_t* s_beg,
                                            uint32_t* s_wsum) {
   const uint32_t b = uniform(off[u]);
@@ -163,16 +166,17 @@ __device__ __forceinline__ void cycle_walk(const uint32_t* __restrict__ off,
       const uint32_t j = upper_index(s_pre, kBlock, f);
       const uint32_t excl = j ? s_pre[j - 1] : 0u;
       const uint32_t x = adj[s_beg[j] + (f - excl)];
-      if (clear)
-        cnt[x] = 0u;
-      else
-        *acc += atomicAdd(&cnt[x], 1u);
+      const uint32_t old = atomicAdd(&cnt[x], 1u);
+      *acc += old;
+      if (old == 0u) touched[atomicAdd(s_nt, 1u)] = x;
     }
     __syncthreads();
   }
 }
 
-// Block per root, highest rank first (dynamic queue).
+// Block per root, highest rank first (dynamic queue).  Each block owns
+// 2 n words at counters + blockIdx.x * stride: n counters, then the list of
+// the counters the current root touched.
 __global__ __launch_bounds__(kBlock) void cycles_kernel(
     const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj, uint32_t n,
     unsigned int* __restrict__ queue, uint32_t* __restrict__ counters, uint64_t stride,
@@ -181,8 +185,11 @@ __global__ __launch_bounds__(kBlock) void cycles_kernel(
   __shared__ uint32_t s_beg[kBlock];
   __shared__ uint32_t s_wsum[kBlock / kWave];
   __shared__ uint32_t s_root;
+  __shared__ uint32_t s_nt;
   uint32_t* cnt = counters + (uint64_t)blockIdx.x * stride;
+  uint32_t* touched = cnt + n;
   uint64_t acc = 0;
+  if (threadIdx.x == 0) s_nt = 0u;
   while (true) {
     if (threadIdx.x == 0) s_root = atomicAdd(queue, 1u);
     __syncthreads();
@@ -190,14 +197,16 @@ __global__ __launch_bounds__(kBlock) void cycles_kernel(
     __syncthreads();
     if (q >= n) break;
     const uint32_t u = n - 1 - q;
-    cycle_walk(off, adj, u, cnt, false, &acc, s_pre, s_beg, s_wsum);
-    // Counter updates of this root must be complete before the clearing
-    // walk writes zeros; the fence orders them.
+    cycle_walk(off, adj, u, cnt, touched, &s_nt, &acc, s_pre, s_beg, s_wsum);
+    // Counter updates and list appends of this root must be complete before
+    // the listed counters are cleared; the fence orders them.
     __threadfence();
     __syncthreads();
-    cycle_walk(off, adj, u, cnt, true, &acc, s_pre, s_beg, s_wsum);
+    const uint32_t nt = uniform(s_nt);
+    for (uint32_t i = threadIdx.x; i < nt; i += kBlock) cnt[touched[i]] = 0u;
     __threadfence();
     __syncthreads();
+    if (threadIdx.x == 0) s_nt = 0u;
   }
   acc = wave_sum_u64(acc);
   if (lane_id() == 0 && acc) atomicAdd(out, (unsigned long long)acc);
@@ -304,10 +313,11 @@ int motif4_by_identities(gpm_graph* g, uint64_t out[6]) {
   if (!g->dag && (st = gpm_graph_orient(g, &g->dag)) != GPM_OK) return st;
   if ((st = ensure_src(g->dag)) != GPM_OK) return st;
   const int cus = device_cus(g);
-  // Scratch: block-private 4-cycle counters (one per vertex), at most 4 GiB.
-  const uint64_t stride = std::max<uint32_t>(g->n, 1);
+  // Scratch: per block, n 4-cycle counters and a list of up to n touched
+  // counters; at most 8 GiB in all.
+  const uint64_t stride = 2 * (uint64_t)std::max<uint32_t>(g->n, 1);
   unsigned blocks_c = (unsigned)std::min<uint64_t>((uint64_t)cus * 2, std::max(g->n, 1u));
-  while (blocks_c > 16 && (uint64_t)blocks_c * stride * 4 > (4ull << 30)) blocks_c /= 2;
+  while (blocks_c > 16 && (uint64_t)blocks_c * stride * 4 > (8ull << 30)) blocks_c /= 2;
   const uint64_t cyc_bytes = (uint64_t)blocks_c * stride * 4;
   if ((st = ensure_scratch(g, cyc_bytes + 256)) != GPM_OK) return st;
   uint32_t* ccnt = static_cast<uint32_t*>(g->scratch);
