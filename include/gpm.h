@@ -266,10 +266,13 @@ typedef struct gpm_query {
 } gpm_query;
 
 typedef struct gpm_tuning {
-  int autotune;             /* 1: project autotuner picks per-query knobs    */
+  int autotune;             /* 1: project autotuner picks per-query chunks;
+                               with policy LEARNED also the plan, by one
+                               trial launch over sampled tasks              */
   int policy;               /* gpm_sched_policy for the matching order      */
   uint32_t chunk;           /* candidate roots per task, 0 = automatic       */
-  int use_graph;            /* 1: capture the batch into one hipGraph        */
+  int use_graph;            /* 1: replay the batch as one hipGraph; needs
+                               autotune = 0 (else GPM_EUNSUPPORTED)          */
   uint32_t reserved[4];
 } gpm_tuning;
 
