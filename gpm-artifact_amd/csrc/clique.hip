@@ -566,18 +566,38 @@ __global__ __launch_bounds__(BS) void clique_block_kernel(
     if ((MODE == kCount || MODE == kAll) && K >= 4) {
       // Narrowest register width that holds this root's rows.
       constexpr bool ALL = MODE == kAll;
-      if (MAXW == 8 && W <= 2)
-        c = count_pairs<K, (MAXW == 8 ? 2 : MAXW), BS, ALL>(rows, d, W, RS, part, n_parts, s_pre,
-                                                            s_wsum, acc3, acc4);
-      else if (MAXW == 8 && W <= 4)
-        c = count_pairs<K, (MAXW == 8 ? 4 : MAXW), BS, ALL>(rows, d, W, RS, part, n_parts, s_pre,
-                                                            s_wsum, acc3, acc4);
-      else if (MAXW == 16 && W <= 12)
-        c = count_pairs<K, (MAXW == 16 ? 12 : MAXW), BS, ALL>(rows, d, W, RS, part, n_parts,
-                                                              s_pre, s_wsum, acc3, acc4);
-      else
+      // One instantiation per word count: the k = 5 loop reads MAXW words
+      // of every row it visits, so a narrower width saves LDS reads.
+#define GPM_PAIRS(NW)                                                                     \
+  count_pairs<K, (MAXW == 8 ? ((NW) <= 8 ? (NW) : 8) : (MAXW == 16 ? (NW) : MAXW)), BS, ALL>( \
+      rows, d, W, RS, part, n_parts, s_pre, s_wsum, acc3, acc4)
+      if (MAXW == 8) {
+        switch (W) {
+          case 0: case 1: case 2: c = GPM_PAIRS(2); break;
+          case 3: c = GPM_PAIRS(3); break;
+          case 4: c = GPM_PAIRS(4); break;
+          case 5: c = GPM_PAIRS(5); break;
+          case 6: c = GPM_PAIRS(6); break;
+          case 7: c = GPM_PAIRS(7); break;
+          default: c = GPM_PAIRS(8); break;
+        }
+      } else if (MAXW == 16) {
+        switch (W) {
+          case 0: case 1: case 2: case 3: case 4: case 5: case 6: case 7: case 8: case 9:
+            c = GPM_PAIRS(9); break;
+          case 10: c = GPM_PAIRS(10); break;
+          case 11: c = GPM_PAIRS(11); break;
+          case 12: c = GPM_PAIRS(12); break;
+          case 13: c = GPM_PAIRS(13); break;
+          case 14: c = GPM_PAIRS(14); break;
+          case 15: c = GPM_PAIRS(15); break;
+          default: c = GPM_PAIRS(16); break;
+        }
+      } else {
         c = count_pairs<K, MAXW, BS, ALL>(rows, d, W, RS, part, n_parts, s_pre, s_wsum, acc3,
                                           acc4);
+      }
+#undef GPM_PAIRS
     } else {
       c = count_rows<K, MODE, BS>(rows, d, W, RS, part, n_parts, lens);
     }
