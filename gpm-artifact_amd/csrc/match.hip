@@ -815,8 +815,9 @@ HostQuery host_query(const gpm_query& gq) {
 
 // Depth-1 entries per part of a heavy root.  Small parts keep one hub's
 // subtree from serialising on a few waves: on labeled R-MAT s22 the batch of
-// four named queries took 9.2 s with parts of 128 entries, 2.2 s with 32 and
-// 2.0 s with 4; below 4 the extra tasks start to cost on light queries.
+// four named queries took 9.2 s with parts of 128 entries, 3.0 s with 32,
+// 2.2 s with 8 and 2.0 s with 4; below 4 the extra tasks start to cost on
+// light queries.
 constexpr uint32_t kSplitLen = 4;
 constexpr uint32_t kTrialUnits = 128;  // tasks a plan trial samples per plan (about)
 
