@@ -820,6 +820,7 @@ HostQuery host_query(const gpm_query& gq) {
 // light queries.
 constexpr uint32_t kSplitLen = 4;
 constexpr uint32_t kTrialUnits = 128;  // tasks a plan trial samples per plan (about)
+constexpr double kTrialMargin = 0.9;   // a trial pick must be 10 % cheaper to replace
 
 uint32_t root_hash(uint32_t v) {
   v ^= v >> 16;
@@ -1109,7 +1110,9 @@ int plan_queries(gpm_graph* g, const gpm_query* queries, int n_queries, const gp
   // Estimated time of a plan in dependent-load rounds: its total work spread
   // over the resident waves, or its largest task, whichever is longer.
   // Plans with fewer (estimated) tasks stopped at the budget rank first --
-  // their other figures are truncated; ties keep the earlier, better
+  // their other figures are truncated; otherwise a later candidate replaces
+  // the current pick only when its estimate is lower by more than the
+  // sampling noise (kTrialMargin), so near-ties keep the earlier, better
   // estimated plan.
   const double waves = (double)t->grid * kWavesPerBlock;
   free_batch(t);
@@ -1122,7 +1125,7 @@ int plan_queries(gpm_graph* g, const gpm_query* queries, int n_queries, const gp
       const double stopped = (double)work[3 * i + 2] * scale[i];
       const double est =
           std::max((double)work[3 * i] * scale[i] / waves, (double)work[3 * i + 1]);
-      if (stopped < best_stopped || (stopped == best_stopped && est < best_est)) {
+      if (stopped < best_stopped || (stopped == best_stopped && est < kTrialMargin * best_est)) {
         best_stopped = stopped;
         best_est = est;
         best = i;
