@@ -1116,6 +1116,12 @@ int plan_queries(gpm_graph* g, const gpm_query* queries, int n_queries, const gp
   // estimated plan.
   const double waves = (double)t->grid * kWavesPerBlock;
   free_batch(t);
+  if (st == GPM_EINVAL || st == GPM_ENOMEM) {
+    // The trial holds every candidate's roots at once; when that does not
+    // fit, the cost-based plans are kept untried.
+    clear_error();
+    return GPM_OK;
+  }
   if (st != GPM_OK) return st;
   for (const Tried& tr : tried) {
     size_t best = tr.first;
