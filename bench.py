@@ -372,6 +372,7 @@ def main():
     achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
     plan_total = {k: dg.clique_plan_bytes(k) for k in ks}
 
+    traffic = measured_traffic(dom, args.scale)
     out = {
         "metric": METRIC,
         "value": value,
@@ -403,7 +404,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dom, args.scale),
+            "traffic": traffic,
+            # Measured HBM bytes over the launch time, as a fraction of the
+            # peak: how far the kernel is from being HBM-bound.
+            "traffic_frac": (traffic / (dom_ms * 1e-3) / (HBM_PEAK_GBS * 1e9)
+                             if traffic and dom_ms else None),
             "algorithmic_bytes_per_launch": alg_bytes,
             "mean_launch_ms": dom_ms,
         },
