@@ -725,25 +725,17 @@ int launch_wave_tier(gpm_graph* g, hipStream_t stream, const RootTiers* t, unsig
   return GPM_OK;
 }
 
-// Launches every non-empty tier.  overlap == false: one stream, tiers one
-// after another.  overlap == true: the rows-in-HBM and wave tiers go to the
-// handle's second stream, so they run in the tails of (and, for the wave
-// tier, beside) the LDS tiers; per-kernel marks then cover the main stream.
+// Launches every non-empty tier on the handle's stream, one after another.
+// (Running the rows-in-HBM and wave tiers on a second stream beside the LDS
+// tiers was measured slower: 254 ms against 230 ms for the learned 5-clique
+// count on R-MAT s22, from contention with the LDS tiers.)
 template <int K, int MODE>
 int launch_tiers(gpm_graph* g, const RootTiers* t, unsigned long long* out,
-                 unsigned long long* work, bool overlap) {
+                 unsigned long long* work) {
   const int cus = device_cus(g);
   unsigned int* q = t->queues;
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, g->stream, q, 4);
   GPM_HIP(hipGetLastError());
-  hipStream_t side = g->stream;
-  if (overlap) {
-    int st = ensure_aux(g);
-    if (st != GPM_OK) return st;
-    GPM_HIP(hipEventRecord(g->ev_fork, g->stream));
-    GPM_HIP(hipStreamWaitEvent(g->aux, g->ev_fork, 0));
-    side = g->aux;
-  }
   if (t->n_big) {
     const uint32_t cap = t->max_big_deg;
     const uint32_t W = (cap + 63) / 64;
@@ -755,14 +747,10 @@ int launch_tiers(gpm_graph* g, const RootTiers* t, unsigned long long* out,
     unsigned long long* grows = static_cast<unsigned long long*>(g->scratch);
     uint32_t* gmeta = reinterpret_cast<uint32_t*>(static_cast<char*>(g->scratch) + row_bytes);
     hipLaunchKernelGGL((clique_block_kernel<K, MODE, false, 0, kHbmBlock>), dim3(blocks),
-                       dim3(kHbmBlock), 0, side, g->toff, g->tadj, t->d_big, t->d_big_parts,
+                       dim3(kHbmBlock), 0, g->stream, g->toff, g->tadj, t->d_big, t->d_big_parts,
                        t->n_big, cap, 0u, q + 2, grows, gmeta, out, work);
     GPM_HIP(hipGetLastError());
-    if ((MODE == kCount || MODE == kAll) && !overlap) timing_mark(g, "clique_hbm_block");
-  }
-  if (overlap && t->n_wave) {
-    int st = launch_wave_tier<K, MODE>(g, side, t, q, out, work);
-    if (st != GPM_OK) return st;
+    if (MODE == kCount || MODE == kAll) timing_mark(g, "clique_hbm_block");
   }
   if (t->n_block2) {
     int st = launch_lds_tier<K, MODE, 16, kWideBlock>(g, g->stream, t->d_block2, t->d_block2_parts,
@@ -776,25 +764,21 @@ int launch_tiers(gpm_graph* g, const RootTiers* t, unsigned long long* out,
                                                     "clique_lds_block");
     if (st != GPM_OK) return st;
   }
-  if (!overlap && t->n_wave) {
+  if (t->n_wave) {
     int st = launch_wave_tier<K, MODE>(g, g->stream, t, q, out, work);
     if (st != GPM_OK) return st;
-  }
-  if (overlap) {
-    GPM_HIP(hipEventRecord(g->ev_join, g->aux));
-    GPM_HIP(hipStreamWaitEvent(g->stream, g->ev_join, 0));
   }
   return GPM_OK;
 }
 
 template <int MODE>
 int launch_k(gpm_graph* g, int k, const RootTiers* t, unsigned long long* out,
-             unsigned long long* work, bool overlap = false) {
-  if (MODE == kAll) return launch_tiers<5, MODE>(g, t, out, work, overlap);
+             unsigned long long* work) {
+  if (MODE == kAll) return launch_tiers<5, MODE>(g, t, out, work);
   switch (k) {
-    case 3: return launch_tiers<3, MODE>(g, t, out, work, overlap);
-    case 4: return launch_tiers<4, MODE>(g, t, out, work, overlap);
-    default: return launch_tiers<5, MODE>(g, t, out, work, overlap);
+    case 3: return launch_tiers<3, MODE>(g, t, out, work);
+    case 4: return launch_tiers<4, MODE>(g, t, out, work);
+    default: return launch_tiers<5, MODE>(g, t, out, work);
   }
 }
 
@@ -851,11 +835,7 @@ int gpm_clique(gpm_graph* g, int k, const gpm_sched* sched, uint64_t* out) {
   if ((st = get_tiers(g, req, &t)) != GPM_OK) return st;
   if ((st = timing_begin(g)) != GPM_OK) return st;
   if ((st = zero_counters(g, 1)) != GPM_OK) return st;
-  // The learned scheduler also overlaps tiers: the HBM-row tier (split into
-  // parts) and the wave tier run on the second stream beside the LDS tiers.
-  // The plain scheduler launches the tiers one after another.
-  const bool overlap = req.policy == GPM_SCHED_LEARNED;
-  if ((st = launch_k<kCount>(g, k, t, g->counters, nullptr, overlap)) != GPM_OK) return st;
+  if ((st = launch_k<kCount>(g, k, t, g->counters, nullptr)) != GPM_OK) return st;
   if ((st = timing_end(g)) != GPM_OK) return st;
   unsigned long long h = 0;
   GPM_HIP(hipMemcpyAsync(&h, g->counters, sizeof(h), hipMemcpyDeviceToHost, g->stream));
@@ -887,8 +867,7 @@ int gpm_clique_all(gpm_graph* g, const gpm_sched* sched, uint64_t out[3]) {
   if ((st = get_tiers(g, req, &t)) != GPM_OK) return st;
   if ((st = timing_begin(g)) != GPM_OK) return st;
   if ((st = zero_counters(g, 3)) != GPM_OK) return st;
-  const bool overlap = req.policy == GPM_SCHED_LEARNED;
-  if ((st = launch_k<kAll>(g, 5, t, g->counters, nullptr, overlap)) != GPM_OK) return st;
+  if ((st = launch_k<kAll>(g, 5, t, g->counters, nullptr)) != GPM_OK) return st;
   if ((st = timing_end(g)) != GPM_OK) return st;
   unsigned long long h[3] = {0, 0, 0};
   GPM_HIP(hipMemcpyAsync(h, g->counters, sizeof(h), hipMemcpyDeviceToHost, g->stream));

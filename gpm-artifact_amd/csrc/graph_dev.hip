@@ -102,14 +102,6 @@ int h2d(gpm_graph* g, void* dst, const void* src, size_t bytes) {
   return GPM_OK;
 }
 
-int ensure_aux(gpm_graph* g) {
-  if (g->aux) return GPM_OK;
-  GPM_HIP(hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming));
-  GPM_HIP(hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming));
-  GPM_HIP(hipStreamCreateWithFlags(&g->aux, hipStreamNonBlocking));
-  return GPM_OK;
-}
-
 int ensure_topo(gpm_graph* g) {
   if (g->toff) return GPM_OK;
   std::vector<uint32_t> h_adj(std::max<uint64_t>(g->m, 1));
@@ -229,12 +221,6 @@ void release(gpm_graph* g) {
   hipFree(g->counters);
   for (auto& e : g->mark_ev)
     if (e) (void)hipEventDestroy(e);
-  if (g->aux) {
-    (void)hipStreamSynchronize(g->aux);
-    (void)hipStreamDestroy(g->aux);
-  }
-  if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
-  if (g->ev_join) (void)hipEventDestroy(g->ev_join);
   if (g->ev0) hipEventDestroy(g->ev0);
   if (g->ev1) hipEventDestroy(g->ev1);
   if (g->stream) hipStreamDestroy(g->stream);

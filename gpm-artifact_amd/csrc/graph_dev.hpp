@@ -37,10 +37,6 @@ struct gpm_graph {
   std::vector<uint32_t> h_labels;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // Second stream (default priority) for kernels that may overlap the main
-  // stream's, with the fork/join events; created on first use.
-  hipStream_t aux = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   double last_ms = 0.0;
   // Per-kernel marks of the last call: an event after each launch.
   static constexpr int kMaxMarks = 8;
@@ -97,8 +93,6 @@ int timing_mark(gpm_graph* g, const char* name);
 int timing_end(gpm_graph* g);
 // Number of compute units of the handle's device.
 int device_cus(const gpm_graph* g);
-// Creates g->aux and its fork/join events if missing.
-int ensure_aux(gpm_graph* g);
 // Builds g->toff/tadj/h_rank if missing; GPM_EINVAL if the graph has a cycle.
 int ensure_topo(gpm_graph* g);
 // Releases the cached matching batch of g (match.hip).
