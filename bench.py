@@ -312,9 +312,9 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the "
-              "multi-GPU split of roots is described in DESIGN.md but not built.",
-              file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); splitting "
+              "the roots over several GPUs (contiguous root ranges, one all-reduce of "
+              "the uint64 counts) is not built.", file=sys.stderr)
         sys.exit(2)
 
     t0 = time.perf_counter()
