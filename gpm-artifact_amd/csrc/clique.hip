@@ -812,85 +812,117 @@ float learned_heavy_cost(const float* cost, uint32_t n);
 extern "C" {
 
 int gpm_clique(gpm_graph* g, int k, const gpm_sched* sched, uint64_t* out) {
-  clear_error();
-  GPM_CHECK_ARG(g && out, "null pointer");
-  GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "k-clique counting needs an ORIENTED graph");
-  GPM_CHECK_ARG(k >= 3 && k <= 5, "k must be 3, 4 or 5 (got %d)", k);
-  GPM_CHECK_ARG(!sched || sched->policy == GPM_SCHED_DEGREE || sched->policy == GPM_SCHED_LEARNED,
-                "unknown scheduling policy");
-  GPM_CHECK_ARG(!sched || sched->policy != GPM_SCHED_LEARNED || sched->root_cost,
-                "LEARNED policy needs root_cost");
-  int st = ensure_topo(g);
-  if (st != GPM_OK) return st;
-  TierRequest req = request_from(sched, (uint32_t)(k - 1));
-  req.rank = g->h_rank.data();
-  if (req.policy == GPM_SCHED_LEARNED) {
-    req.heavy_cost = learned_heavy_cost(req.cost, g->n);
-    const uint32_t cus = (uint32_t)device_cus(g);
-    req.resident_block = cus * lds_blocks_per_cu(req.block_cap, kLdsBlock, true);
-    req.resident_block2 = cus * lds_blocks_per_cu(req.block2_cap, kWideBlock, false);
-    req.resident_big = cus;
-  }
-  const RootTiers* t = nullptr;
-  if ((st = get_tiers(g, req, &t)) != GPM_OK) return st;
-  if ((st = timing_begin(g)) != GPM_OK) return st;
-  if ((st = zero_counters(g, 1)) != GPM_OK) return st;
-  if ((st = launch_k<kCount>(g, k, t, g->counters, nullptr)) != GPM_OK) return st;
-  if ((st = timing_end(g)) != GPM_OK) return st;
-  unsigned long long h = 0;
-  GPM_HIP(hipMemcpyAsync(&h, g->counters, sizeof(h), hipMemcpyDeviceToHost, g->stream));
-  GPM_HIP(hipStreamSynchronize(g->stream));
-  *out = h;
-  return GPM_OK;
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    GPM_CHECK_ARG(g && out, "null pointer");
+    GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "k-clique counting needs an ORIENTED graph");
+    GPM_CHECK_ARG(k >= 3 && k <= 5, "k must be 3, 4 or 5 (got %d)", k);
+    GPM_CHECK_ARG(!sched || sched->policy == GPM_SCHED_DEGREE || sched->policy == GPM_SCHED_LEARNED,
+                  "unknown scheduling policy");
+    GPM_CHECK_ARG(!sched || sched->policy != GPM_SCHED_LEARNED || sched->root_cost,
+                  "LEARNED policy needs root_cost");
+    int st = ensure_topo(g);
+    if (st != GPM_OK) return st;
+    TierRequest req = request_from(sched, (uint32_t)(k - 1));
+    req.rank = g->h_rank.data();
+    if (req.policy == GPM_SCHED_LEARNED) {
+      req.heavy_cost = learned_heavy_cost(req.cost, g->n);
+      const uint32_t cus = (uint32_t)device_cus(g);
+      req.resident_block = cus * lds_blocks_per_cu(req.block_cap, kLdsBlock, true);
+      req.resident_block2 = cus * lds_blocks_per_cu(req.block2_cap, kWideBlock, false);
+      req.resident_big = cus;
+    }
+    const RootTiers* t = nullptr;
+    if ((st = get_tiers(g, req, &t)) != GPM_OK) return st;
+    if ((st = timing_begin(g)) != GPM_OK) return st;
+    if ((st = zero_counters(g, 1)) != GPM_OK) return st;
+    if ((st = launch_k<kCount>(g, k, t, g->counters, nullptr)) != GPM_OK) return st;
+    if ((st = timing_end(g)) != GPM_OK) return st;
+    unsigned long long h = 0;
+    GPM_HIP(hipMemcpyAsync(&h, g->counters, sizeof(h), hipMemcpyDeviceToHost, g->stream));
+    GPM_HIP(hipStreamSynchronize(g->stream));
+    *out = h;
+    return GPM_OK;
+  });
 }
 
 int gpm_clique_all(gpm_graph* g, const gpm_sched* sched, uint64_t out[3]) {
-  clear_error();
-  GPM_CHECK_ARG(g && out, "null pointer");
-  GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "k-clique counting needs an ORIENTED graph");
-  GPM_CHECK_ARG(!sched || sched->policy == GPM_SCHED_DEGREE || sched->policy == GPM_SCHED_LEARNED,
-                "unknown scheduling policy");
-  GPM_CHECK_ARG(!sched || sched->policy != GPM_SCHED_LEARNED || sched->root_cost,
-                "LEARNED policy needs root_cost");
-  int st = ensure_topo(g);
-  if (st != GPM_OK) return st;
-  TierRequest req = request_from(sched, 2u);  // triangles need out-degree >= 2
-  req.rank = g->h_rank.data();
-  if (req.policy == GPM_SCHED_LEARNED) {
-    req.heavy_cost = learned_heavy_cost(req.cost, g->n);
-    const uint32_t cus = (uint32_t)device_cus(g);
-    req.resident_block = cus * lds_blocks_per_cu(req.block_cap, kLdsBlock, true);
-    req.resident_block2 = cus * lds_blocks_per_cu(req.block2_cap, kWideBlock, false);
-    req.resident_big = cus;
-  }
-  const RootTiers* t = nullptr;
-  if ((st = get_tiers(g, req, &t)) != GPM_OK) return st;
-  if ((st = timing_begin(g)) != GPM_OK) return st;
-  if ((st = zero_counters(g, 3)) != GPM_OK) return st;
-  if ((st = launch_k<kAll>(g, 5, t, g->counters, nullptr)) != GPM_OK) return st;
-  if ((st = timing_end(g)) != GPM_OK) return st;
-  unsigned long long h[3] = {0, 0, 0};
-  GPM_HIP(hipMemcpyAsync(h, g->counters, sizeof(h), hipMemcpyDeviceToHost, g->stream));
-  GPM_HIP(hipStreamSynchronize(g->stream));
-  for (int i = 0; i < 3; ++i) out[i] = h[i];
-  return GPM_OK;
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    GPM_CHECK_ARG(g && out, "null pointer");
+    GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "k-clique counting needs an ORIENTED graph");
+    GPM_CHECK_ARG(!sched || sched->policy == GPM_SCHED_DEGREE || sched->policy == GPM_SCHED_LEARNED,
+                  "unknown scheduling policy");
+    GPM_CHECK_ARG(!sched || sched->policy != GPM_SCHED_LEARNED || sched->root_cost,
+                  "LEARNED policy needs root_cost");
+    int st = ensure_topo(g);
+    if (st != GPM_OK) return st;
+    TierRequest req = request_from(sched, 2u);  // triangles need out-degree >= 2
+    req.rank = g->h_rank.data();
+    if (req.policy == GPM_SCHED_LEARNED) {
+      req.heavy_cost = learned_heavy_cost(req.cost, g->n);
+      const uint32_t cus = (uint32_t)device_cus(g);
+      req.resident_block = cus * lds_blocks_per_cu(req.block_cap, kLdsBlock, true);
+      req.resident_block2 = cus * lds_blocks_per_cu(req.block2_cap, kWideBlock, false);
+      req.resident_big = cus;
+    }
+    const RootTiers* t = nullptr;
+    if ((st = get_tiers(g, req, &t)) != GPM_OK) return st;
+    if ((st = timing_begin(g)) != GPM_OK) return st;
+    if ((st = zero_counters(g, 3)) != GPM_OK) return st;
+    if ((st = launch_k<kAll>(g, 5, t, g->counters, nullptr)) != GPM_OK) return st;
+    if ((st = timing_end(g)) != GPM_OK) return st;
+    unsigned long long h[3] = {0, 0, 0};
+    GPM_HIP(hipMemcpyAsync(h, g->counters, sizeof(h), hipMemcpyDeviceToHost, g->stream));
+    GPM_HIP(hipStreamSynchronize(g->stream));
+    for (int i = 0; i < 3; ++i) out[i] = h[i];
+    return GPM_OK;
+  });
 }
 
 int gpm_clique_tier_caps(uint32_t* wave_cap, uint32_t* lds_cap, uint32_t* wide_cap) {
-  clear_error();
-  GPM_CHECK_ARG(wave_cap && lds_cap && wide_cap, "null pointer");
-  *wave_cap = 64;
-  *lds_cap = kBlockCap;
-  *wide_cap = lds_wide_cap();
-  return GPM_OK;
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    GPM_CHECK_ARG(wave_cap && lds_cap && wide_cap, "null pointer");
+    *wave_cap = 64;
+    *lds_cap = kBlockCap;
+    *wide_cap = lds_wide_cap();
+    return GPM_OK;
+  });
 }
 
 int gpm_clique_plan_bytes(gpm_graph* g, int k, uint64_t* bytes) {
-  clear_error();
-  GPM_CHECK_ARG(g && bytes, "null pointer");
-  GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "needs an ORIENTED graph");
-  GPM_CHECK_ARG(k >= 3 && k <= 5, "k must be 3, 4 or 5 (got %d)", k);
-  if (!g->plan_bytes[k]) {
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    GPM_CHECK_ARG(g && bytes, "null pointer");
+    GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "needs an ORIENTED graph");
+    GPM_CHECK_ARG(k >= 3 && k <= 5, "k must be 3, 4 or 5 (got %d)", k);
+    if (!g->plan_bytes[k]) {
+      int st = ensure_topo(g);
+      if (st != GPM_OK) return st;
+      TierRequest req = request_from(nullptr, 1);
+      req.rank = g->h_rank.data();
+      const RootTiers* t = nullptr;
+      st = get_tiers(g, req, &t);
+      if (st != GPM_OK) return st;
+      if ((st = zero_counters(g, 1)) != GPM_OK) return st;
+      if ((st = launch_k<kBytes>(g, k, t, g->counters, nullptr)) != GPM_OK) return st;
+      unsigned long long h = 0;
+      GPM_HIP(hipMemcpyAsync(&h, g->counters, sizeof(h), hipMemcpyDeviceToHost, g->stream));
+      GPM_HIP(hipStreamSynchronize(g->stream));
+      g->plan_bytes[k] = h + 4ull * (g->n + 1) + 8ull * g->m;
+    }
+    *bytes = g->plan_bytes[k];
+    return GPM_OK;
+  });
+}
+
+int gpm_clique_root_work(gpm_graph* g, int k, uint64_t* work) {
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    GPM_CHECK_ARG(g && work, "null pointer");
+    GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "needs an ORIENTED graph");
+    GPM_CHECK_ARG(k >= 3 && k <= 5, "k must be 3, 4 or 5 (got %d)", k);
     int st = ensure_topo(g);
     if (st != GPM_OK) return st;
     TierRequest req = request_from(nullptr, 1);
@@ -898,48 +930,26 @@ int gpm_clique_plan_bytes(gpm_graph* g, int k, uint64_t* bytes) {
     const RootTiers* t = nullptr;
     st = get_tiers(g, req, &t);
     if (st != GPM_OK) return st;
-    if ((st = zero_counters(g, 1)) != GPM_OK) return st;
-    if ((st = launch_k<kBytes>(g, k, t, g->counters, nullptr)) != GPM_OK) return st;
-    unsigned long long h = 0;
-    GPM_HIP(hipMemcpyAsync(&h, g->counters, sizeof(h), hipMemcpyDeviceToHost, g->stream));
-    GPM_HIP(hipStreamSynchronize(g->stream));
-    g->plan_bytes[k] = h + 4ull * (g->n + 1) + 8ull * g->m;
-  }
-  *bytes = g->plan_bytes[k];
-  return GPM_OK;
-}
-
-int gpm_clique_root_work(gpm_graph* g, int k, uint64_t* work) {
-  clear_error();
-  GPM_CHECK_ARG(g && work, "null pointer");
-  GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "needs an ORIENTED graph");
-  GPM_CHECK_ARG(k >= 3 && k <= 5, "k must be 3, 4 or 5 (got %d)", k);
-  int st = ensure_topo(g);
-  if (st != GPM_OK) return st;
-  TierRequest req = request_from(nullptr, 1);
-  req.rank = g->h_rank.data();
-  const RootTiers* t = nullptr;
-  st = get_tiers(g, req, &t);
-  if (st != GPM_OK) return st;
-  unsigned long long* d_work = nullptr;
-  GPM_HIP(hipMalloc(&d_work, (size_t)std::max(g->n, 1u) * 8));
-  if (hipMemsetAsync(d_work, 0, (size_t)std::max(g->n, 1u) * 8, g->stream) != hipSuccess) {
-    (void)hipFree(d_work);
-    return fail(GPM_EHIP, "clearing the work log failed");
-  }
-  st = launch_k<kWork>(g, k, t, g->counters, d_work);
-  // The kernels log by topological id; report by vertex id.
-  std::vector<uint64_t> by_rank(g->n);
-  if (st == GPM_OK) {
-    if (hipMemcpyAsync(by_rank.data(), d_work, (size_t)g->n * 8, hipMemcpyDeviceToHost,
-                       g->stream) != hipSuccess ||
-        hipStreamSynchronize(g->stream) != hipSuccess)
-      st = fail(GPM_EHIP, "work log copy failed");
-  }
-  hipFree(d_work);
-  if (st == GPM_OK)
-    for (uint32_t v = 0; v < g->n; ++v) work[v] = by_rank[g->h_rank[v]];
-  return st;
+    unsigned long long* d_work = nullptr;
+    GPM_HIP(hipMalloc(&d_work, (size_t)std::max(g->n, 1u) * 8));
+    if (hipMemsetAsync(d_work, 0, (size_t)std::max(g->n, 1u) * 8, g->stream) != hipSuccess) {
+      (void)hipFree(d_work);
+      return fail(GPM_EHIP, "clearing the work log failed");
+    }
+    st = launch_k<kWork>(g, k, t, g->counters, d_work);
+    // The kernels log by topological id; report by vertex id.
+    std::vector<uint64_t> by_rank(g->n);
+    if (st == GPM_OK) {
+      if (hipMemcpyAsync(by_rank.data(), d_work, (size_t)g->n * 8, hipMemcpyDeviceToHost,
+                         g->stream) != hipSuccess ||
+          hipStreamSynchronize(g->stream) != hipSuccess)
+        st = fail(GPM_EHIP, "work log copy failed");
+    }
+    hipFree(d_work);
+    if (st == GPM_OK)
+      for (uint32_t v = 0; v < g->n; ++v) work[v] = by_rank[g->h_rank[v]];
+    return st;
+  });
 }
 
 }  // extern "C"

@@ -7,6 +7,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -59,6 +62,24 @@ void build_label_csr(uint32_t n, uint32_t L, const uint32_t* offsets,
 bool topo_relabel(uint32_t n, const uint32_t* offsets, const uint32_t* adj,
                   std::vector<uint32_t>& rank, std::vector<uint32_t>& new_off,
                   std::vector<uint32_t>& new_adj);
+
+// Runs the body of a C-ABI entry point: a C++ exception must not cross the
+// C boundary, so host allocation failures become GPM_ENOMEM and any other
+// exception GPM_EINVAL, each with a message.
+template <typename F>
+int guarded(F&& body) {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    return fail(GPM_ENOMEM, "host allocation failed");
+  } catch (const std::length_error&) {
+    return fail(GPM_ENOMEM, "host allocation too large");
+  } catch (const std::exception& e) {
+    return fail(GPM_EINVAL, "%s", e.what());
+  } catch (...) {
+    return fail(GPM_EINVAL, "unexpected exception");
+  }
+}
 
 template <typename T>
 T* xmalloc(size_t count) {

@@ -269,188 +269,200 @@ extern "C" {
 int gpm_graph_create(const uint32_t* offsets, uint32_t n, const uint32_t* adj,
                      uint64_t m, const uint32_t* labels, uint32_t n_labels,
                      uint32_t flags, gpm_graph** out) {
-  clear_error();
-  GPM_CHECK_ARG(out && offsets, "null pointer");
-  GPM_CHECK_ARG(m == 0 || adj, "null adjacency");
-  GPM_CHECK_ARG(n < 0xFFFFFFFFu, "n must be < 2^32 - 1");
-  GPM_CHECK_ARG(m < (1ull << 32), "m must be < 2^32");
-  GPM_CHECK_ARG(offsets[0] == 0 && offsets[n] == m, "offsets[0] must be 0 and offsets[n] == m");
-  GPM_CHECK_ARG(!labels || n_labels >= 1, "labels given with n_labels == 0");
-  GPM_CHECK_ARG(n_labels <= 4096, "n_labels must be <= 4096");
-  *out = nullptr;
-  // Host-side validation so that no kernel can index out of range.
-  uint32_t maxd = 0;
-  for (uint32_t v = 0; v < n; ++v) {
-    if (offsets[v + 1] < offsets[v]) return fail(GPM_EINVAL, "offsets not monotone at %u", v);
-    maxd = std::max(maxd, offsets[v + 1] - offsets[v]);
-  }
-  for (uint64_t i = 0; i < m; ++i)
-    if (adj[i] >= n) return fail(GPM_EINVAL, "adjacency entry %llu = %u >= n",
-                                 (unsigned long long)i, adj[i]);
-  if (labels)
-    for (uint32_t v = 0; v < n; ++v)
-      if (labels[v] >= n_labels) return fail(GPM_EINVAL, "label of %u >= n_labels", v);
-  int dev = 0;
-  GPM_HIP(hipGetDevice(&dev));
-  gpm_graph* g = new gpm_graph();
-  g->device = dev;
-  g->n = n;
-  g->m = m;
-  g->max_deg = maxd;
-  g->flags = flags;
-  g->n_labels = labels ? n_labels : 0;
-  g->h_off.assign(offsets, offsets + n + 1);
-  if (labels) g->h_labels.assign(labels, labels + n);
-  int st = GPM_OK;
-  do {
-    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess) {
-      st = fail(GPM_EHIP, "stream/event creation failed");
-      break;
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    GPM_CHECK_ARG(out && offsets, "null pointer");
+    GPM_CHECK_ARG(m == 0 || adj, "null adjacency");
+    GPM_CHECK_ARG(n < 0xFFFFFFFFu, "n must be < 2^32 - 1");
+    GPM_CHECK_ARG(m < (1ull << 32), "m must be < 2^32");
+    GPM_CHECK_ARG(offsets[0] == 0 && offsets[n] == m, "offsets[0] must be 0 and offsets[n] == m");
+    GPM_CHECK_ARG(!labels || n_labels >= 1, "labels given with n_labels == 0");
+    GPM_CHECK_ARG(n_labels <= 4096, "n_labels must be <= 4096");
+    *out = nullptr;
+    // Host-side validation so that no kernel can index out of range.
+    uint32_t maxd = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+      if (offsets[v + 1] < offsets[v]) return fail(GPM_EINVAL, "offsets not monotone at %u", v);
+      maxd = std::max(maxd, offsets[v + 1] - offsets[v]);
     }
-    if (hipMalloc(&g->off, (size_t)(n + 1) * 4) != hipSuccess ||
-        hipMalloc(&g->adj, std::max<uint64_t>(m, 1) * 4) != hipSuccess ||
-        hipMalloc(&g->counters, 64 * sizeof(unsigned long long)) != hipSuccess) {
-      st = fail(GPM_ENOMEM, "device CSR allocation (n=%u, m=%llu)", n, (unsigned long long)m);
-      break;
-    }
-    if (hipMemcpyAsync(g->off, offsets, (size_t)(n + 1) * 4, hipMemcpyHostToDevice,
-                       g->stream) != hipSuccess ||
-        (m && hipMemcpyAsync(g->adj, adj, m * 4, hipMemcpyHostToDevice, g->stream) !=
-                  hipSuccess)) {
-      st = fail(GPM_EHIP, "H2D copy of the CSR failed");
-      break;
-    }
-    if (labels) {
-      if (hipMalloc(&g->labels, (size_t)std::max(n, 1u) * 4) != hipSuccess) {
-        st = fail(GPM_ENOMEM, "device labels");
+    for (uint64_t i = 0; i < m; ++i)
+      if (adj[i] >= n) return fail(GPM_EINVAL, "adjacency entry %llu = %u >= n",
+                                   (unsigned long long)i, adj[i]);
+    if (labels)
+      for (uint32_t v = 0; v < n; ++v)
+        if (labels[v] >= n_labels) return fail(GPM_EINVAL, "label of %u >= n_labels", v);
+    int dev = 0;
+    GPM_HIP(hipGetDevice(&dev));
+    gpm_graph* g = new gpm_graph();
+    g->device = dev;
+    g->n = n;
+    g->m = m;
+    g->max_deg = maxd;
+    g->flags = flags;
+    g->n_labels = labels ? n_labels : 0;
+    g->h_off.assign(offsets, offsets + n + 1);
+    if (labels) g->h_labels.assign(labels, labels + n);
+    int st = GPM_OK;
+    do {
+      if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess) {
+        st = fail(GPM_EHIP, "stream/event creation failed");
         break;
       }
-      if (hipMemcpyAsync(g->labels, labels, (size_t)n * 4, hipMemcpyHostToDevice,
-                         g->stream) != hipSuccess) {
-        st = fail(GPM_EHIP, "H2D copy of labels failed");
+      if (hipMalloc(&g->off, (size_t)(n + 1) * 4) != hipSuccess ||
+          hipMalloc(&g->adj, std::max<uint64_t>(m, 1) * 4) != hipSuccess ||
+          hipMalloc(&g->counters, 64 * sizeof(unsigned long long)) != hipSuccess) {
+        st = fail(GPM_ENOMEM, "device CSR allocation (n=%u, m=%llu)", n, (unsigned long long)m);
         break;
       }
-      st = build_label_index(g, offsets, adj, labels);
-      if (st != GPM_OK) break;
+      if (hipMemcpyAsync(g->off, offsets, (size_t)(n + 1) * 4, hipMemcpyHostToDevice,
+                         g->stream) != hipSuccess ||
+          (m && hipMemcpyAsync(g->adj, adj, m * 4, hipMemcpyHostToDevice, g->stream) !=
+                    hipSuccess)) {
+        st = fail(GPM_EHIP, "H2D copy of the CSR failed");
+        break;
+      }
+      if (labels) {
+        if (hipMalloc(&g->labels, (size_t)std::max(n, 1u) * 4) != hipSuccess) {
+          st = fail(GPM_ENOMEM, "device labels");
+          break;
+        }
+        if (hipMemcpyAsync(g->labels, labels, (size_t)n * 4, hipMemcpyHostToDevice,
+                           g->stream) != hipSuccess) {
+          st = fail(GPM_EHIP, "H2D copy of labels failed");
+          break;
+        }
+        st = build_label_index(g, offsets, adj, labels);
+        if (st != GPM_OK) break;
+      }
+      if (hipStreamSynchronize(g->stream) != hipSuccess) {
+        st = fail(GPM_EHIP, "stream synchronize failed");
+        break;
+      }
+    } while (0);
+    if (st != GPM_OK) {
+      release(g);
+      return st;
     }
-    if (hipStreamSynchronize(g->stream) != hipSuccess) {
-      st = fail(GPM_EHIP, "stream synchronize failed");
-      break;
-    }
-  } while (0);
-  if (st != GPM_OK) {
-    release(g);
-    return st;
-  }
-  *out = g;
-  return GPM_OK;
+    *out = g;
+    return GPM_OK;
+  });
 }
 
 int gpm_graph_destroy(gpm_graph* g) {
-  clear_error();
-  release(g);
-  return GPM_OK;
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    release(g);
+    return GPM_OK;
+  });
 }
 
 int gpm_graph_info(const gpm_graph* g, uint32_t* n, uint64_t* m, uint32_t* max_deg) {
-  clear_error();
-  GPM_CHECK_ARG(g, "null graph");
-  if (n) *n = g->n;
-  if (m) *m = g->m;
-  if (max_deg) *max_deg = g->max_deg;
-  return GPM_OK;
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    GPM_CHECK_ARG(g, "null graph");
+    if (n) *n = g->n;
+    if (m) *m = g->m;
+    if (max_deg) *max_deg = g->max_deg;
+    return GPM_OK;
+  });
 }
 
 int gpm_last_kernel_ms(const gpm_graph* g, double* ms) {
-  clear_error();
-  GPM_CHECK_ARG(g && ms, "null pointer");
-  *ms = g->last_ms;
-  return GPM_OK;
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    GPM_CHECK_ARG(g && ms, "null pointer");
+    *ms = g->last_ms;
+    return GPM_OK;
+  });
 }
 
 int gpm_last_kernel_breakdown(const gpm_graph* g, double* ms, const char** names,
                               int capacity, int* count) {
-  clear_error();
-  GPM_CHECK_ARG(g && count, "null pointer");
-  const int n = std::min(capacity, g->n_marks);
-  for (int i = 0; i < n; ++i) {
-    if (ms) ms[i] = g->mark_ms[i];
-    if (names) names[i] = g->mark_name[i];
-  }
-  *count = g->n_marks;
-  return GPM_OK;
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    GPM_CHECK_ARG(g && count, "null pointer");
+    const int n = std::min(capacity, g->n_marks);
+    for (int i = 0; i < n; ++i) {
+      if (ms) ms[i] = g->mark_ms[i];
+      if (names) names[i] = g->mark_name[i];
+    }
+    *count = g->n_marks;
+    return GPM_OK;
+  });
 }
 
 int gpm_graph_orient(gpm_graph* sym, gpm_graph** dag) {
-  clear_error();
-  GPM_CHECK_ARG(sym && dag, "null pointer");
-  GPM_CHECK_ARG(sym->flags & GPM_GRAPH_SYMMETRIC, "graph is not SYMMETRIC");
-  const uint32_t n = sym->n;
-  uint32_t* cnt = nullptr;
-  GPM_HIP(hipMalloc(&cnt, (size_t)(n + 1) * 4));
-  const int grid = grid_for(sym, (uint64_t)n * kWave, 256);
-  int st = timing_begin(sym);
-  if (st != GPM_OK) {
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    GPM_CHECK_ARG(sym && dag, "null pointer");
+    GPM_CHECK_ARG(sym->flags & GPM_GRAPH_SYMMETRIC, "graph is not SYMMETRIC");
+    const uint32_t n = sym->n;
+    uint32_t* cnt = nullptr;
+    GPM_HIP(hipMalloc(&cnt, (size_t)(n + 1) * 4));
+    const int grid = grid_for(sym, (uint64_t)n * kWave, 256);
+    int st = timing_begin(sym);
+    if (st != GPM_OK) {
+      hipFree(cnt);
+      return st;
+    }
+    hipLaunchKernelGGL(orient_count_kernel, dim3(grid), dim3(256), 0, sym->stream, sym->off,
+                       sym->adj, n, cnt);
+    std::vector<uint32_t> h_cnt(n + 1, 0);
+    if (hipMemcpyAsync(h_cnt.data(), cnt, (size_t)n * 4, hipMemcpyDeviceToHost, sym->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(sym->stream) != hipSuccess) {
+      hipFree(cnt);
+      return fail(GPM_EHIP, "orientation count pass failed");
+    }
+    // Exclusive scan on the host (n+1 entries); the list pass is on the GPU.
+    std::vector<uint32_t> h_new(n + 1, 0);
+    uint32_t maxd = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+      h_new[v + 1] = h_new[v] + h_cnt[v];
+      maxd = std::max(maxd, h_cnt[v]);
+    }
+    const uint64_t m2 = h_new[n];
+    gpm_graph* g = new gpm_graph();
+    g->device = sym->device;
+    g->n = n;
+    g->m = m2;
+    g->max_deg = maxd;
+    g->flags = GPM_GRAPH_ORIENTED;
+    g->h_off = h_new;
+    do {
+      if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess) {
+        st = fail(GPM_EHIP, "stream/event creation failed");
+        break;
+      }
+      if (hipMalloc(&g->off, (size_t)(n + 1) * 4) != hipSuccess ||
+          hipMalloc(&g->adj, std::max<uint64_t>(m2, 1) * 4) != hipSuccess ||
+          hipMalloc(&g->counters, 64 * sizeof(unsigned long long)) != hipSuccess) {
+        st = fail(GPM_ENOMEM, "device CSR allocation");
+        break;
+      }
+      if (hipMemcpyAsync(g->off, h_new.data(), (size_t)(n + 1) * 4, hipMemcpyHostToDevice,
+                         sym->stream) != hipSuccess) {
+        st = fail(GPM_EHIP, "H2D of offsets failed");
+        break;
+      }
+      hipLaunchKernelGGL(orient_fill_kernel, dim3(grid), dim3(256), 0, sym->stream, sym->off,
+                         sym->adj, n, g->off, g->adj);
+      if (hipGetLastError() != hipSuccess) {
+        st = fail(GPM_EHIP, "orientation fill launch failed");
+        break;
+      }
+      st = timing_end(sym);
+    } while (0);
     hipFree(cnt);
-    return st;
-  }
-  hipLaunchKernelGGL(orient_count_kernel, dim3(grid), dim3(256), 0, sym->stream, sym->off,
-                     sym->adj, n, cnt);
-  std::vector<uint32_t> h_cnt(n + 1, 0);
-  if (hipMemcpyAsync(h_cnt.data(), cnt, (size_t)n * 4, hipMemcpyDeviceToHost, sym->stream) !=
-          hipSuccess ||
-      hipStreamSynchronize(sym->stream) != hipSuccess) {
-    hipFree(cnt);
-    return fail(GPM_EHIP, "orientation count pass failed");
-  }
-  // Exclusive scan on the host (n+1 entries); the list pass is on the GPU.
-  std::vector<uint32_t> h_new(n + 1, 0);
-  uint32_t maxd = 0;
-  for (uint32_t v = 0; v < n; ++v) {
-    h_new[v + 1] = h_new[v] + h_cnt[v];
-    maxd = std::max(maxd, h_cnt[v]);
-  }
-  const uint64_t m2 = h_new[n];
-  gpm_graph* g = new gpm_graph();
-  g->device = sym->device;
-  g->n = n;
-  g->m = m2;
-  g->max_deg = maxd;
-  g->flags = GPM_GRAPH_ORIENTED;
-  g->h_off = h_new;
-  do {
-    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess) {
-      st = fail(GPM_EHIP, "stream/event creation failed");
-      break;
+    if (st != GPM_OK) {
+      release(g);
+      return st;
     }
-    if (hipMalloc(&g->off, (size_t)(n + 1) * 4) != hipSuccess ||
-        hipMalloc(&g->adj, std::max<uint64_t>(m2, 1) * 4) != hipSuccess ||
-        hipMalloc(&g->counters, 64 * sizeof(unsigned long long)) != hipSuccess) {
-      st = fail(GPM_ENOMEM, "device CSR allocation");
-      break;
-    }
-    if (hipMemcpyAsync(g->off, h_new.data(), (size_t)(n + 1) * 4, hipMemcpyHostToDevice,
-                       sym->stream) != hipSuccess) {
-      st = fail(GPM_EHIP, "H2D of offsets failed");
-      break;
-    }
-    hipLaunchKernelGGL(orient_fill_kernel, dim3(grid), dim3(256), 0, sym->stream, sym->off,
-                       sym->adj, n, g->off, g->adj);
-    if (hipGetLastError() != hipSuccess) {
-      st = fail(GPM_EHIP, "orientation fill launch failed");
-      break;
-    }
-    st = timing_end(sym);
-  } while (0);
-  hipFree(cnt);
-  if (st != GPM_OK) {
-    release(g);
-    return st;
-  }
-  *dag = g;
-  return GPM_OK;
+    *dag = g;
+    return GPM_OK;
+  });
 }
 
 }  // extern "C"

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <deque>
 #include <vector>
 
 #include "device_common.hpp"
@@ -49,7 +50,8 @@ struct gpm_graph {
   size_t scratch_bytes = 0;
   unsigned long long* counters = nullptr;
   // Root tiers built for earlier calls, keyed by the request that made them.
-  std::vector<std::pair<std::string, gpm::RootTiers>> tier_cache;
+  // A deque keeps pointers to cached tiers valid while entries are added.
+  std::deque<std::pair<gpm::TierKey, gpm::RootTiers>> tier_cache;
   // Cached algorithmic-byte totals per k (0 = not computed yet).
   uint64_t plan_bytes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   // Symmetric graphs: the graph relabelled by degree rank (position in

@@ -704,7 +704,7 @@ bool connected(const HostQuery& q) {
 // graph handle, so calling again with the same queries and tuning replays
 // the graph without planning, filtering or tuning again.
 struct MatchBatch {
-  uint64_t key = 0;
+  std::vector<uint32_t> key;
   void* mem = nullptr;
   QPlan* d_plans = nullptr;
   uint32_t* d_roots = nullptr;
@@ -733,21 +733,23 @@ void free_batch(void* p) {
   delete b;
 }
 
-uint64_t batch_key(const gpm_query* queries, int n_queries, const gpm_tuning& t) {
-  uint64_t h = 1469598103934665603ull;
-  auto mixin = [&](uint64_t x) { h = (h ^ x) * 1099511628211ull; };
-  mixin((uint64_t)n_queries);
-  mixin((uint64_t)t.autotune);
-  mixin((uint64_t)t.policy);
-  mixin((uint64_t)t.chunk);
-  mixin((uint64_t)t.use_graph);
+// Full key material of a batch request: tuning fields, then per query its
+// size, edge list and labels, as one word sequence.  A cached batch is
+// replayed only when its stored sequence equals the new request's.
+std::vector<uint32_t> batch_key(const gpm_query* queries, int n_queries, const gpm_tuning& t) {
+  std::vector<uint32_t> k;
+  k.push_back((uint32_t)n_queries);
+  k.push_back((uint32_t)t.autotune);
+  k.push_back((uint32_t)t.policy);
+  k.push_back(t.chunk);
+  k.push_back((uint32_t)t.use_graph);
   for (int i = 0; i < n_queries; ++i) {
-    mixin(queries[i].n);
-    mixin(queries[i].n_edges);
-    for (uint32_t e = 0; e < 2 * queries[i].n_edges; ++e) mixin(queries[i].edges[e]);
-    for (uint32_t v = 0; v < queries[i].n; ++v) mixin(queries[i].labels[v]);
+    k.push_back(queries[i].n);
+    k.push_back(queries[i].n_edges);
+    k.insert(k.end(), queries[i].edges, queries[i].edges + 2 * (size_t)queries[i].n_edges);
+    k.insert(k.end(), queries[i].labels, queries[i].labels + queries[i].n);
   }
-  return h;
+  return k;
 }
 
 // Enqueues counter reset + enumeration of a prepared batch.
@@ -1162,65 +1164,67 @@ void free_match_cache(gpm_graph* g) {
 
 extern "C" int gpm_match_batch(gpm_graph* g, const gpm_query* queries, int n_queries,
                                const gpm_tuning* tuning, uint64_t* per_query) {
-  clear_error();
-  GPM_CHECK_ARG(g && per_query && (queries || n_queries == 0), "null pointer");
-  GPM_CHECK_ARG(n_queries >= 0, "negative query count");
-  GPM_CHECK_ARG(g->flags & GPM_GRAPH_SYMMETRIC, "matching needs a SYMMETRIC graph");
-  GPM_CHECK_ARG(g->labels && g->ladj, "matching needs a labeled graph");
-  gpm_tuning tun{};
-  tun.autotune = 1;
-  tun.use_graph = 0;
-  if (tuning) tun = *tuning;
-  GPM_CHECK_ARG(tun.policy == GPM_SCHED_DEGREE || tun.policy == GPM_SCHED_LEARNED,
-                "unknown policy %d", tun.policy);
-  // Autotuned batches are not captured into a hipGraph: that combination
-  // stopped responding on the device with the earlier probe-based tuner and
-  // stays disabled until its cause is known.  Tuned batches run as one
-  // persistent launch; graph replay is available with fixed chunks.
-  if (tun.autotune && tun.use_graph)
-    return fail(GPM_EUNSUPPORTED, "autotune with use_graph is not supported; "
-                                  "set autotune = 0 (fixed chunk) or use_graph = 0");
-  for (int qi = 0; qi < n_queries; ++qi) {
-    const gpm_query& gq = queries[qi];
-    GPM_CHECK_ARG(gq.n >= 1 && gq.n <= (uint32_t)kMaxQ, "query %d: 1 <= n <= 16 (got %u)", qi,
-                  gq.n);
-    GPM_CHECK_ARG(gq.labels && (gq.n_edges == 0 || gq.edges), "query %d: null arrays", qi);
-    HostQuery hq;
-    hq.n = gq.n;
-    hq.adj.assign(gq.n, {});
-    for (uint32_t e = 0; e < gq.n_edges; ++e) {
-      const uint32_t a = gq.edges[2 * e], c = gq.edges[2 * e + 1];
-      GPM_CHECK_ARG(a < gq.n && c < gq.n && a != c, "query %d: bad edge %u (%u, %u)", qi, e, a, c);
-      hq.adj[a].push_back(c);
-      hq.adj[c].push_back(a);
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    GPM_CHECK_ARG(g && per_query && (queries || n_queries == 0), "null pointer");
+    GPM_CHECK_ARG(n_queries >= 0, "negative query count");
+    GPM_CHECK_ARG(g->flags & GPM_GRAPH_SYMMETRIC, "matching needs a SYMMETRIC graph");
+    GPM_CHECK_ARG(g->labels && g->ladj, "matching needs a labeled graph");
+    gpm_tuning tun{};
+    tun.autotune = 1;
+    tun.use_graph = 0;
+    if (tuning) tun = *tuning;
+    GPM_CHECK_ARG(tun.policy == GPM_SCHED_DEGREE || tun.policy == GPM_SCHED_LEARNED,
+                  "unknown policy %d", tun.policy);
+    // Autotuned batches are not captured into a hipGraph: that combination
+    // stopped responding on the device with the earlier probe-based tuner and
+    // stays disabled until its cause is known.  Tuned batches run as one
+    // persistent launch; graph replay is available with fixed chunks.
+    if (tun.autotune && tun.use_graph)
+      return fail(GPM_EUNSUPPORTED, "autotune with use_graph is not supported; "
+                                    "set autotune = 0 (fixed chunk) or use_graph = 0");
+    for (int qi = 0; qi < n_queries; ++qi) {
+      const gpm_query& gq = queries[qi];
+      GPM_CHECK_ARG(gq.n >= 1 && gq.n <= (uint32_t)kMaxQ, "query %d: 1 <= n <= 16 (got %u)", qi,
+                    gq.n);
+      GPM_CHECK_ARG(gq.labels && (gq.n_edges == 0 || gq.edges), "query %d: null arrays", qi);
+      HostQuery hq;
+      hq.n = gq.n;
+      hq.adj.assign(gq.n, {});
+      for (uint32_t e = 0; e < gq.n_edges; ++e) {
+        const uint32_t a = gq.edges[2 * e], c = gq.edges[2 * e + 1];
+        GPM_CHECK_ARG(a < gq.n && c < gq.n && a != c, "query %d: bad edge %u (%u, %u)", qi, e, a, c);
+        hq.adj[a].push_back(c);
+        hq.adj[c].push_back(a);
+      }
+      GPM_CHECK_ARG(connected(hq), "query %d is not connected", qi);
     }
-    GPM_CHECK_ARG(connected(hq), "query %d is not connected", qi);
-  }
-  for (int i = 0; i < n_queries; ++i) per_query[i] = 0;
-  if (n_queries == 0) return GPM_OK;
-  const uint64_t key = batch_key(queries, n_queries, tun);
-  MatchBatch* b = static_cast<MatchBatch*>(g->match_cache);
-  if (!b || b->key != key) {
-    free_match_cache(g);
-    b = new MatchBatch();
-    int st = prepare_batch(g, queries, n_queries, tun, b);
-    if (st != GPM_OK) {
-      free_batch(b);
-      return st;
+    for (int i = 0; i < n_queries; ++i) per_query[i] = 0;
+    if (n_queries == 0) return GPM_OK;
+    std::vector<uint32_t> key = batch_key(queries, n_queries, tun);
+    MatchBatch* b = static_cast<MatchBatch*>(g->match_cache);
+    if (!b || b->key != key) {
+      free_match_cache(g);
+      b = new MatchBatch();
+      int st = prepare_batch(g, queries, n_queries, tun, b);
+      if (st != GPM_OK) {
+        free_batch(b);
+        return st;
+      }
+      b->key.swap(key);
+      g->match_cache = b;
     }
-    b->key = key;
-    g->match_cache = b;
-  }
-  if (b->nq == 0) return GPM_OK;
-  int st;
-  if ((st = timing_begin(g)) != GPM_OK) return st;
-  if ((st = launch_batch(g, b)) != GPM_OK) return st;
-  timing_mark(g, "match_batch");
-  if ((st = timing_end(g)) != GPM_OK) return st;
-  std::vector<unsigned long long> cnt(n_queries);
-  GPM_HIP(hipMemcpyAsync(cnt.data(), b->d_counts, (size_t)n_queries * 8, hipMemcpyDeviceToHost,
-                         g->stream));
-  GPM_HIP(hipStreamSynchronize(g->stream));
-  for (int i = 0; i < n_queries; ++i) per_query[i] = cnt[i];
-  return GPM_OK;
+    if (b->nq == 0) return GPM_OK;
+    int st;
+    if ((st = timing_begin(g)) != GPM_OK) return st;
+    if ((st = launch_batch(g, b)) != GPM_OK) return st;
+    timing_mark(g, "match_batch");
+    if ((st = timing_end(g)) != GPM_OK) return st;
+    std::vector<unsigned long long> cnt(n_queries);
+    GPM_HIP(hipMemcpyAsync(cnt.data(), b->d_counts, (size_t)n_queries * 8, hipMemcpyDeviceToHost,
+                           g->stream));
+    GPM_HIP(hipStreamSynchronize(g->stream));
+    for (int i = 0; i < n_queries; ++i) per_query[i] = cnt[i];
+    return GPM_OK;
+  });
 }

@@ -171,88 +171,92 @@ __global__ __launch_bounds__(kBlock) void features_kernel(
 extern "C" {
 
 int gpm_vertex_features(gpm_graph* sym, gpm_graph* dag, float* feat) {
-  clear_error();
-  GPM_CHECK_ARG(sym && feat, "null pointer");
-  GPM_CHECK_ARG(!dag || dag->n == sym->n, "sym and dag differ in n");
-  const uint32_t n = sym->n;
-  float* d_feat = nullptr;
-  GPM_HIP(hipMalloc(&d_feat, (size_t)std::max(n, 1u) * 16));
-  const int cus = device_cus(sym);
-  const unsigned blocks = (unsigned)std::max<uint64_t>(
-      1, std::min<uint64_t>(((uint64_t)n * kWave + kBlock - 1) / kBlock, (uint64_t)cus * 16));
-  hipLaunchKernelGGL(features_kernel, dim3(blocks), dim3(kBlock), 0, sym->stream, sym->off,
-                     sym->adj, dag ? dag->off : nullptr, dag ? dag->adj : nullptr, sym->labels, n,
-                     d_feat);
-  hipError_t e = hipGetLastError();
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(feat, d_feat, (size_t)n * 16, hipMemcpyDeviceToHost, sym->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(sym->stream);
-  (void)hipFree(d_feat);
-  if (e != hipSuccess) return fail(GPM_EHIP, "vertex features: %s", hipGetErrorString(e));
-  return GPM_OK;
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    GPM_CHECK_ARG(sym && feat, "null pointer");
+    GPM_CHECK_ARG(!dag || dag->n == sym->n, "sym and dag differ in n");
+    const uint32_t n = sym->n;
+    float* d_feat = nullptr;
+    GPM_HIP(hipMalloc(&d_feat, (size_t)std::max(n, 1u) * 16));
+    const int cus = device_cus(sym);
+    const unsigned blocks = (unsigned)std::max<uint64_t>(
+        1, std::min<uint64_t>(((uint64_t)n * kWave + kBlock - 1) / kBlock, (uint64_t)cus * 16));
+    hipLaunchKernelGGL(features_kernel, dim3(blocks), dim3(kBlock), 0, sym->stream, sym->off,
+                       sym->adj, dag ? dag->off : nullptr, dag ? dag->adj : nullptr, sym->labels, n,
+                       d_feat);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(feat, d_feat, (size_t)n * 16, hipMemcpyDeviceToHost, sym->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(sym->stream);
+    (void)hipFree(d_feat);
+    if (e != hipSuccess) return fail(GPM_EHIP, "vertex features: %s", hipGetErrorString(e));
+    return GPM_OK;
+  });
 }
 
 int gpm_mlp_predict(const float* feat, uint32_t n, const float* norm_mean, const float* norm_std,
                     const float* w1, const float* b1, const float* w2, const float* b2,
                     const float* w3, const float* b3, float* out) {
-  clear_error();
-  GPM_CHECK_ARG(feat && norm_mean && norm_std && w1 && b1 && w2 && b2 && w3 && b3 && out,
-                "null pointer");
-  for (int k = 0; k < kFeat; ++k)
-    GPM_CHECK_ARG(norm_std[k] > 0.f && std::isfinite(norm_std[k]) && std::isfinite(norm_mean[k]),
-                  "normalisation std must be finite and > 0");
-  for (uint32_t i = 0; i < 4 * n; ++i)
-    GPM_CHECK_ARG(feat[i] > -1.f && std::isfinite(feat[i]), "features must be finite and > -1");
-  const size_t wcount = 4 + 4 + 64 * 4 + 64 + 64 * 64 + 64 + 64 + 1;
-  std::vector<float> packed;
-  packed.reserve(wcount);
-  auto put = [&](const float* p, size_t c) { packed.insert(packed.end(), p, p + c); };
-  put(norm_mean, 4);
-  put(norm_std, 4);
-  put(w1, 64 * 4);
-  put(b1, 64);
-  put(w2, 64 * 64);
-  put(b2, 64);
-  put(w3, 64);
-  put(b3, 1);
-  int dev = 0;
-  GPM_HIP(hipGetDevice(&dev));
-  float *d_w = nullptr, *d_feat = nullptr, *d_out = nullptr;
-  hipStream_t stream = nullptr;
-  hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipMalloc(&d_w, wcount * 4);
-  if (e == hipSuccess) e = hipMalloc(&d_feat, (size_t)std::max(n, 1u) * 16);
-  if (e == hipSuccess) e = hipMalloc(&d_out, (size_t)std::max(n, 1u) * 4);
-  if (e == hipSuccess) e = hipMemcpyAsync(d_w, packed.data(), wcount * 4, hipMemcpyHostToDevice, stream);
-  if (e == hipSuccess && n)
-    e = hipMemcpyAsync(d_feat, feat, (size_t)n * 16, hipMemcpyHostToDevice, stream);
-  if (e == hipSuccess && n) {
-    Mlp m;
-    m.mean = d_w;
-    m.std = d_w + 4;
-    m.w1 = d_w + 8;
-    m.b1 = m.w1 + 64 * 4;
-    m.w2 = m.b1 + 64;
-    m.b2 = m.w2 + 64 * 64;
-    m.w3 = m.b2 + 64;
-    m.b3 = m.w3 + 64;
-    hipDeviceProp_t prop;
-    int cus = 256;
-    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
-    const uint64_t groups = (n + 15) / 16;
-    const unsigned blocks = (unsigned)std::max<uint64_t>(
-        1, std::min<uint64_t>((groups + kWavesPerBlock - 1) / kWavesPerBlock, (uint64_t)cus * 8));
-    hipLaunchKernelGGL(mlp_kernel, dim3(blocks), dim3(kBlock), 0, stream, d_feat, n, m, d_out);
-    e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, (size_t)n * 4, hipMemcpyDeviceToHost, stream);
-  }
-  if (e == hipSuccess) e = hipStreamSynchronize(stream);
-  (void)hipFree(d_w);
-  (void)hipFree(d_feat);
-  (void)hipFree(d_out);
-  if (stream) (void)hipStreamDestroy(stream);
-  if (e != hipSuccess) return fail(GPM_EHIP, "mlp predict: %s", hipGetErrorString(e));
-  return GPM_OK;
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    GPM_CHECK_ARG(feat && norm_mean && norm_std && w1 && b1 && w2 && b2 && w3 && b3 && out,
+                  "null pointer");
+    for (int k = 0; k < kFeat; ++k)
+      GPM_CHECK_ARG(norm_std[k] > 0.f && std::isfinite(norm_std[k]) && std::isfinite(norm_mean[k]),
+                    "normalisation std must be finite and > 0");
+    for (uint32_t i = 0; i < 4 * n; ++i)
+      GPM_CHECK_ARG(feat[i] > -1.f && std::isfinite(feat[i]), "features must be finite and > -1");
+    const size_t wcount = 4 + 4 + 64 * 4 + 64 + 64 * 64 + 64 + 64 + 1;
+    std::vector<float> packed;
+    packed.reserve(wcount);
+    auto put = [&](const float* p, size_t c) { packed.insert(packed.end(), p, p + c); };
+    put(norm_mean, 4);
+    put(norm_std, 4);
+    put(w1, 64 * 4);
+    put(b1, 64);
+    put(w2, 64 * 64);
+    put(b2, 64);
+    put(w3, 64);
+    put(b3, 1);
+    int dev = 0;
+    GPM_HIP(hipGetDevice(&dev));
+    float *d_w = nullptr, *d_feat = nullptr, *d_out = nullptr;
+    hipStream_t stream = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&d_w, wcount * 4);
+    if (e == hipSuccess) e = hipMalloc(&d_feat, (size_t)std::max(n, 1u) * 16);
+    if (e == hipSuccess) e = hipMalloc(&d_out, (size_t)std::max(n, 1u) * 4);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_w, packed.data(), wcount * 4, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess && n)
+      e = hipMemcpyAsync(d_feat, feat, (size_t)n * 16, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess && n) {
+      Mlp m;
+      m.mean = d_w;
+      m.std = d_w + 4;
+      m.w1 = d_w + 8;
+      m.b1 = m.w1 + 64 * 4;
+      m.w2 = m.b1 + 64;
+      m.b2 = m.w2 + 64 * 64;
+      m.w3 = m.b2 + 64;
+      m.b3 = m.w3 + 64;
+      hipDeviceProp_t prop;
+      int cus = 256;
+      if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+      const uint64_t groups = (n + 15) / 16;
+      const unsigned blocks = (unsigned)std::max<uint64_t>(
+          1, std::min<uint64_t>((groups + kWavesPerBlock - 1) / kWavesPerBlock, (uint64_t)cus * 8));
+      hipLaunchKernelGGL(mlp_kernel, dim3(blocks), dim3(kBlock), 0, stream, d_feat, n, m, d_out);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, (size_t)n * 4, hipMemcpyDeviceToHost, stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    (void)hipFree(d_w);
+    (void)hipFree(d_feat);
+    (void)hipFree(d_out);
+    if (stream) (void)hipStreamDestroy(stream);
+    if (e != hipSuccess) return fail(GPM_EHIP, "mlp predict: %s", hipGetErrorString(e));
+    return GPM_OK;
+  });
 }
 
 }  // extern "C"

@@ -284,41 +284,43 @@ int motif4_by_identities(gpm_graph* g, uint64_t out[6]);
 
 extern "C" int gpm_pattern(gpm_graph* g, int pattern_id, int formulation, uint64_t out[6],
                            uint64_t* scalar) {
-  clear_error();
-  GPM_CHECK_ARG(g && out && scalar, "null pointer");
-  GPM_CHECK_ARG(g->flags & GPM_GRAPH_SYMMETRIC, "pattern counting needs a SYMMETRIC graph");
-  GPM_CHECK_ARG(pattern_id >= GPM_P1 && pattern_id <= GPM_MOTIF4, "%d not support!", pattern_id);
-  GPM_CHECK_ARG(formulation == GPM_FORM_G2MINER || formulation == GPM_FORM_PATTERN,
-                "unknown formulation %d", formulation);
-  int st;
-  uint64_t h[6] = {0, 0, 0, 0, 0, 0};
-  const bool identities = formulation == GPM_FORM_PATTERN &&
-                          (pattern_id == GPM_MOTIF4 || pattern_id == GPM_P1 || pattern_id == GPM_P4);
-  if (identities) {
-    uint64_t m4[6];
-    if ((st = motif4_by_identities(g, m4)) != GPM_OK) return st;
-    if (pattern_id == GPM_MOTIF4)
-      for (int i = 0; i < 6; ++i) h[i] = m4[i];
-    else
-      h[0] = pattern_id == GPM_P1 ? m4[0] : m4[1];
-  } else {
-    if ((st = timing_begin(g)) != GPM_OK) return st;
-    if ((st = zero_counters(g, 8)) != GPM_OK) return st;
-    if ((st = launch_direct_pid(g, pattern_id, g->counters)) != GPM_OK) return st;
-    if ((st = timing_end(g)) != GPM_OK) return st;
-    unsigned long long hh[6];
-    GPM_HIP(hipMemcpyAsync(hh, g->counters, sizeof(hh), hipMemcpyDeviceToHost, g->stream));
-    GPM_HIP(hipStreamSynchronize(g->stream));
-    for (int i = 0; i < 6; ++i) h[i] = hh[i];
-  }
-  for (int i = 0; i < 6; ++i) out[i] = h[i];
-  if (pattern_id == GPM_MOTIF4) {
-    // G2MinerSolver returns counters[0] (3-star, G2MinerSolver.cuh:166);
-    // PatternSolver returns h[1] - h[2] (PatternSolver.cuh:251-254), which
-    // after its Motif4 sequence is 4-path minus tailed-triangle.
-    *scalar = formulation == GPM_FORM_G2MINER ? h[0] : h[1] - h[2];
-  } else {
-    *scalar = h[0];
-  }
-  return GPM_OK;
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    GPM_CHECK_ARG(g && out && scalar, "null pointer");
+    GPM_CHECK_ARG(g->flags & GPM_GRAPH_SYMMETRIC, "pattern counting needs a SYMMETRIC graph");
+    GPM_CHECK_ARG(pattern_id >= GPM_P1 && pattern_id <= GPM_MOTIF4, "%d not support!", pattern_id);
+    GPM_CHECK_ARG(formulation == GPM_FORM_G2MINER || formulation == GPM_FORM_PATTERN,
+                  "unknown formulation %d", formulation);
+    int st;
+    uint64_t h[6] = {0, 0, 0, 0, 0, 0};
+    const bool identities = formulation == GPM_FORM_PATTERN &&
+                            (pattern_id == GPM_MOTIF4 || pattern_id == GPM_P1 || pattern_id == GPM_P4);
+    if (identities) {
+      uint64_t m4[6];
+      if ((st = motif4_by_identities(g, m4)) != GPM_OK) return st;
+      if (pattern_id == GPM_MOTIF4)
+        for (int i = 0; i < 6; ++i) h[i] = m4[i];
+      else
+        h[0] = pattern_id == GPM_P1 ? m4[0] : m4[1];
+    } else {
+      if ((st = timing_begin(g)) != GPM_OK) return st;
+      if ((st = zero_counters(g, 8)) != GPM_OK) return st;
+      if ((st = launch_direct_pid(g, pattern_id, g->counters)) != GPM_OK) return st;
+      if ((st = timing_end(g)) != GPM_OK) return st;
+      unsigned long long hh[6];
+      GPM_HIP(hipMemcpyAsync(hh, g->counters, sizeof(hh), hipMemcpyDeviceToHost, g->stream));
+      GPM_HIP(hipStreamSynchronize(g->stream));
+      for (int i = 0; i < 6; ++i) h[i] = hh[i];
+    }
+    for (int i = 0; i < 6; ++i) out[i] = h[i];
+    if (pattern_id == GPM_MOTIF4) {
+      // G2MinerSolver returns counters[0] (3-star, G2MinerSolver.cuh:166);
+      // PatternSolver returns h[1] - h[2] (PatternSolver.cuh:251-254), which
+      // after its Motif4 sequence is 4-path minus tailed-triangle.
+      *scalar = formulation == GPM_FORM_G2MINER ? h[0] : h[1] - h[2];
+    } else {
+      *scalar = h[0];
+    }
+    return GPM_OK;
+  });
 }

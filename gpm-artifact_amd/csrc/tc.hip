@@ -372,13 +372,17 @@ int tc_run(gpm_graph* g, int variant, int reps, uint64_t* out) {
 extern "C" {
 
 int gpm_tc(gpm_graph* g, int variant, uint64_t* out) {
-  clear_error();
-  return tc_run(g, variant, 1, out);
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    return tc_run(g, variant, 1, out);
+  });
 }
 
 int gpm_tc_repeat(gpm_graph* g, int variant, int reps, uint64_t* out) {
-  clear_error();
-  return tc_run(g, variant, reps, out);
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    return tc_run(g, variant, reps, out);
+  });
 }
 
 }  // extern "C"

@@ -10,24 +10,27 @@ namespace gpm {
 
 namespace {
 
-uint64_t checksum(const float* p, uint32_t n) {
-  uint64_t h = 1469598103934665603ull;
-  const uint32_t* w = reinterpret_cast<const uint32_t*>(p);
-  for (uint32_t i = 0; i < n; ++i) h = (h ^ w[i]) * 1099511628211ull;
-  return h;
+// Full key material of a tier request: the scalar fields as text, plus
+// copies of the cost vector (learned policy) and the explicit root list.
+// Compared whole on every lookup, so two requests share tiers only when
+// they are identical.
+TierKey key_of(const TierRequest& r, uint32_t n) {
+  TierKey k;
+  char buf[256];
+  std::snprintf(buf, sizeof(buf), "%u/%u/%u/%u/%u/%u/%d/%u/%a/%u/%u/%u/%d", r.min_deg,
+                r.wave_cap, r.block_cap, r.block2_cap, r.root_begin, r.root_end, r.policy, r.chunk,
+                (double)r.heavy_cost, r.resident_block, r.resident_block2, r.resident_big,
+                r.rank != nullptr);
+  k.params = buf;
+  if (r.policy == GPM_SCHED_LEARNED && r.cost) k.cost.assign(r.cost, r.cost + n);
+  if (r.n_root_list) k.roots.assign(r.root_list, r.root_list + r.n_root_list);
+  return k;
 }
 
-std::string key_of(const TierRequest& r, uint32_t n) {
-  char buf[256];
-  uint64_t cs = (r.policy == GPM_SCHED_LEARNED && r.cost) ? checksum(r.cost, n) : 0;
-  if (r.n_root_list)
-    cs ^= checksum(reinterpret_cast<const float*>(r.root_list), r.n_root_list) * 31 +
-          r.n_root_list;
-  std::snprintf(buf, sizeof(buf), "%u/%u/%u/%u/%u/%u/%d/%u/%g/%u/%u/%u/%d/%llx", r.min_deg,
-                r.wave_cap, r.block_cap, r.block2_cap, r.root_begin, r.root_end, r.policy, r.chunk,
-                r.heavy_cost, r.resident_block, r.resident_block2, r.resident_big,
-                r.rank != nullptr, (unsigned long long)cs);
-  return buf;
+bool same_key(const TierKey& a, const TierKey& b) {
+  return a.params == b.params && a.roots == b.roots && a.cost.size() == b.cost.size() &&
+         (a.cost.empty() ||
+          std::memcmp(a.cost.data(), b.cost.data(), a.cost.size() * sizeof(float)) == 0);
 }
 
 int upload(gpm_graph* g, const std::vector<uint32_t>& v, uint32_t** d) {
@@ -49,9 +52,9 @@ void release(RootTiers& t) {
 }  // namespace
 
 int get_tiers(gpm_graph* g, const TierRequest& req, const RootTiers** out) {
-  const std::string key = key_of(req, g->n);
+  TierKey key = key_of(req, g->n);
   for (auto& kv : g->tier_cache)
-    if (kv.first == key) {
+    if (same_key(kv.first, key)) {
       *out = &kv.second;
       return GPM_OK;
     }
@@ -181,7 +184,7 @@ int get_tiers(gpm_graph* g, const TierRequest& req, const RootTiers** out) {
     release(t);
     return fail(GPM_ENOMEM, "work queues");
   }
-  g->tier_cache.emplace_back(key, t);
+  g->tier_cache.emplace_back(std::move(key), t);
   *out = &g->tier_cache.back().second;
   // Keep the cache small: learned requests with changing costs would grow it.
   if (g->tier_cache.size() > 16) {

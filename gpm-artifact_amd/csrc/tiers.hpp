@@ -37,6 +37,13 @@ struct RootTiers {
   uint32_t wave_chunk = 8;         // roots per wave fetch
 };
 
+// What a cached RootTiers was built for (see key_of in tiers.hip).
+struct TierKey {
+  std::string params;
+  std::vector<float> cost;
+  std::vector<uint32_t> roots;
+};
+
 struct TierRequest {
   uint32_t min_deg = 1;       // roots below this out-degree do nothing
   uint32_t wave_cap = 64;     // max out-degree of a wave root

@@ -117,3 +117,58 @@ def random_queries(count: int, seed: int, min_size: int = 4, max_size: int = 10,
         e = np.asarray(sorted(edges), dtype=np.uint32).reshape(-1, 2)
         out.append((n, e, rng.integers(0, n_labels, n).astype(np.uint32)))
     return out
+
+
+def dense_queries(g: G.HostCSR, labels: np.ndarray, count: int, seed: int,
+                  min_size: int = 4, max_size: int = 10, min_avg_degree: float = 3.0,
+                  max_degree: int = 1 << 30):
+    """Dense connected queries cut out of g (the shape of the reference's
+    query_dense sets: average degree >= 3 where the size allows it).
+
+    A size is drawn uniformly in [min_size, max_size] and a start vertex of
+    degree in [2, max_degree]; the query grows by the data neighbour of its
+    members that has the most edges into the members so far (ties broken at
+    random), and keeps every data edge among its members and their labels.
+    A draw whose average degree stays below min(min_avg_degree, size - 1)
+    is discarded.  Every query has at least one embedding (its own vertices).
+    Deterministic for a given seed.
+    """
+    rng = np.random.default_rng(seed)
+    deg = g.degrees()
+    ok = np.nonzero((deg >= 2) & (deg <= max_degree))[0]
+    out = []
+    while len(out) < count:
+        size = int(rng.integers(min_size, max_size + 1))
+        start = int(ok[rng.integers(0, len(ok))])
+        members = [start]
+        inside = {start}
+        links = {}  # candidate -> edges into the members
+        for w in g.adj[g.offsets[start]:g.offsets[start + 1]]:
+            w = int(w)
+            if deg[w] <= max_degree:
+                links[w] = links.get(w, 0) + 1
+        while len(members) < size and links:
+            best = max(links.values())
+            cands = sorted(v for v, c in links.items() if c == best)
+            b = cands[int(rng.integers(0, len(cands)))]
+            del links[b]
+            members.append(b)
+            inside.add(b)
+            for w in g.adj[g.offsets[b]:g.offsets[b + 1]]:
+                w = int(w)
+                if w not in inside and deg[w] <= max_degree:
+                    links[w] = links.get(w, 0) + 1
+        if len(members) < size:
+            continue
+        idx = {v: i for i, v in enumerate(members)}
+        edges = []
+        for v in members:
+            for w in g.adj[g.offsets[v]:g.offsets[v + 1]]:
+                w = int(w)
+                if w in idx and v < w:
+                    edges.append((idx[v], idx[w]))
+        if 2 * len(edges) < min(min_avg_degree, size - 1) * size:
+            continue
+        e = np.asarray(sorted(edges), dtype=np.uint32).reshape(-1, 2)
+        out.append((size, e, labels[np.asarray(members)].astype(np.uint32)))
+    return out

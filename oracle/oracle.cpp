@@ -15,6 +15,10 @@
  *                     orientation/relabel invariance          tests/test_oracle.py
  *   orc_clique        closed forms (K_n, Turan graphs), networkx clique
  *                     enumeration                             tests/test_oracle.py
+ *   orc_clique_list   per-root closed form on K_n (root v of the id-ordered
+ *                     DAG roots C(n-1-v, k-1) cliques), equality with
+ *                     orc_clique over contiguous ranges and partition sums of
+ *                     seeded random root sets            tests/test_oracle.py
  *   orc_plan_bytes    brute-force recount of the plan's set sizes in Python
  *   orc_motif4,       brute-force induced 4/5-vertex subgraph classification
  *   orc_pattern       (networkx isomorphism) and the identities listed next
@@ -23,7 +27,8 @@
  *                     expected_output.res on HPRD), networkx subgraph
  *                     monomorphism counts, and the reference CECI binary
  *                     built from source (oracle/_ref)         tests/test_ceci.py
- *   orc_match_dfs     same checks as orc_ceci
+ *   orc_ceci_batch    orc_ceci per query, parallel across queries; the 200
+ *                     known answers are checked through it    tests/test_ceci.py
  */
 #include <omp.h>
 

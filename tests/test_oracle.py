@@ -150,3 +150,30 @@ def test_census_sanity():
     for a, b in [(0, 1), (1, 2), (2, 3)]:
         P[a, b] = P[b, a] = 1
     assert brute.census(P, 4) == {brute.code_of(4, brute.SHAPES4["path4"]): 1}
+
+
+@pytest.mark.parametrize("n", [6, 11, 20])
+def test_clique_list_complete_graph_per_root(gpm, orc, n):
+    # K_n oriented by (degree, id): every degree is equal, so v -> w iff w > v
+    # and root v roots exactly the cliques whose smallest id is v:
+    # C(n-1-v, k-1) of them.
+    d = dag(gpm, graphs.complete(n))
+    for k in (3, 4, 5):
+        for v in range(n):
+            assert orc.clique_list(d.offsets, d.adj, k, [v]) == comb(n - 1 - v, k - 1)
+
+
+def test_clique_list_matches_ranges_and_partitions(gpm, orc):
+    g = gpm.rmat_graph(10, 16, 5)
+    d = dag(gpm, g)
+    rng = np.random.default_rng(3)
+    for k in (3, 4, 5):
+        total = orc.clique(d.offsets, d.adj, k)
+        assert orc.clique_list(d.offsets, d.adj, k, np.arange(d.n)) == total
+        assert orc.clique_list(d.offsets, d.adj, k, np.arange(100, 517)) == \
+            orc.clique(d.offsets, d.adj, k, 100, 517)
+        # A seeded random split of the roots into three sets sums to the total.
+        part = rng.integers(0, 3, d.n)
+        parts = [orc.clique_list(d.offsets, d.adj, k, np.nonzero(part == i)[0]) for i in range(3)]
+        assert sum(parts) == total
+        assert orc.clique_list(d.offsets, d.adj, k, []) == 0
