@@ -809,6 +809,45 @@ namespace gpm {
 float learned_heavy_cost(const float* cost, uint32_t n);
 }
 
+namespace {
+
+// Tier request of one k-clique call.  LEARNED uses the caller's predicted
+// costs; DEGREE_SPLIT uses the same ordering, promotion and splitting with
+// the plain degree estimate cost(v) = d+(v)^2 (the root's pair count), so
+// the two differ only in where the cost comes from; DEGREE keeps the
+// reference-style degree order without splitting.
+int prepare_request(gpm_graph* g, const gpm_sched* sched, uint32_t min_deg, TierRequest* req,
+                    std::vector<float>* deg_cost) {
+  GPM_CHECK_ARG(!sched || sched->policy == GPM_SCHED_DEGREE ||
+                    sched->policy == GPM_SCHED_LEARNED || sched->policy == GPM_SCHED_DEGREE_SPLIT,
+                "unknown scheduling policy");
+  GPM_CHECK_ARG(!sched || sched->policy != GPM_SCHED_LEARNED || sched->root_cost,
+                "LEARNED policy needs root_cost");
+  int st = ensure_topo(g);
+  if (st != GPM_OK) return st;
+  *req = request_from(sched, min_deg);
+  req->rank = g->h_rank.data();
+  if (req->policy == GPM_SCHED_DEGREE_SPLIT) {
+    deg_cost->resize(g->n);
+    for (uint32_t v = 0; v < g->n; ++v) {
+      const float d = (float)(g->h_off[v + 1] - g->h_off[v]);
+      (*deg_cost)[v] = d * d;
+    }
+    req->cost = deg_cost->data();
+    req->policy = GPM_SCHED_LEARNED;
+  }
+  if (req->policy == GPM_SCHED_LEARNED) {
+    req->heavy_cost = learned_heavy_cost(req->cost, g->n);
+    const uint32_t cus = (uint32_t)device_cus(g);
+    req->resident_block = cus * lds_blocks_per_cu(req->block_cap, kLdsBlock, true);
+    req->resident_block2 = cus * lds_blocks_per_cu(req->block2_cap, kWideBlock, false);
+    req->resident_big = cus;
+  }
+  return GPM_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int gpm_clique(gpm_graph* g, int k, const gpm_sched* sched, uint64_t* out) {
@@ -817,21 +856,10 @@ int gpm_clique(gpm_graph* g, int k, const gpm_sched* sched, uint64_t* out) {
     GPM_CHECK_ARG(g && out, "null pointer");
     GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "k-clique counting needs an ORIENTED graph");
     GPM_CHECK_ARG(k >= 3 && k <= 5, "k must be 3, 4 or 5 (got %d)", k);
-    GPM_CHECK_ARG(!sched || sched->policy == GPM_SCHED_DEGREE || sched->policy == GPM_SCHED_LEARNED,
-                  "unknown scheduling policy");
-    GPM_CHECK_ARG(!sched || sched->policy != GPM_SCHED_LEARNED || sched->root_cost,
-                  "LEARNED policy needs root_cost");
-    int st = ensure_topo(g);
+    TierRequest req;
+    std::vector<float> deg_cost;
+    int st = prepare_request(g, sched, (uint32_t)(k - 1), &req, &deg_cost);
     if (st != GPM_OK) return st;
-    TierRequest req = request_from(sched, (uint32_t)(k - 1));
-    req.rank = g->h_rank.data();
-    if (req.policy == GPM_SCHED_LEARNED) {
-      req.heavy_cost = learned_heavy_cost(req.cost, g->n);
-      const uint32_t cus = (uint32_t)device_cus(g);
-      req.resident_block = cus * lds_blocks_per_cu(req.block_cap, kLdsBlock, true);
-      req.resident_block2 = cus * lds_blocks_per_cu(req.block2_cap, kWideBlock, false);
-      req.resident_big = cus;
-    }
     const RootTiers* t = nullptr;
     if ((st = get_tiers(g, req, &t)) != GPM_OK) return st;
     if ((st = timing_begin(g)) != GPM_OK) return st;
@@ -851,21 +879,10 @@ int gpm_clique_all(gpm_graph* g, const gpm_sched* sched, uint64_t out[3]) {
     clear_error();
     GPM_CHECK_ARG(g && out, "null pointer");
     GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "k-clique counting needs an ORIENTED graph");
-    GPM_CHECK_ARG(!sched || sched->policy == GPM_SCHED_DEGREE || sched->policy == GPM_SCHED_LEARNED,
-                  "unknown scheduling policy");
-    GPM_CHECK_ARG(!sched || sched->policy != GPM_SCHED_LEARNED || sched->root_cost,
-                  "LEARNED policy needs root_cost");
-    int st = ensure_topo(g);
+    TierRequest req;
+    std::vector<float> deg_cost;
+    int st = prepare_request(g, sched, 2u, &req, &deg_cost);  // triangles need d+ >= 2
     if (st != GPM_OK) return st;
-    TierRequest req = request_from(sched, 2u);  // triangles need out-degree >= 2
-    req.rank = g->h_rank.data();
-    if (req.policy == GPM_SCHED_LEARNED) {
-      req.heavy_cost = learned_heavy_cost(req.cost, g->n);
-      const uint32_t cus = (uint32_t)device_cus(g);
-      req.resident_block = cus * lds_blocks_per_cu(req.block_cap, kLdsBlock, true);
-      req.resident_block2 = cus * lds_blocks_per_cu(req.block2_cap, kWideBlock, false);
-      req.resident_big = cus;
-    }
     const RootTiers* t = nullptr;
     if ((st = get_tiers(g, req, &t)) != GPM_OK) return st;
     if ((st = timing_begin(g)) != GPM_OK) return st;

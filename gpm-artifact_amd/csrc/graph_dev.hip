@@ -215,8 +215,10 @@ void release(gpm_graph* g) {
   hipFree(g->ladj);
   hipFree(g->loff);
   hipFree(g->lverts);
+  (void)hipFree(g->sig);
   (void)hipFree(g->toff);
   (void)hipFree(g->tadj);
+  (void)hipFree(g->bmask);
   hipFree(g->scratch);
   hipFree(g->counters);
   for (auto& e : g->mark_ev)
@@ -248,6 +250,21 @@ int build_label_index(gpm_graph* g, const uint32_t* offsets, const uint32_t* adj
       double* row = g->h_lpair.data() + (size_t)labels[v] * L;
       for (uint32_t l = 0; l < L; ++l) row[l] += lo[l + 1] - lo[l];
     }
+  }
+  // Packed filter signature for small label alphabets: degree in the high
+  // word, per label the neighbour count saturated at 15 in 4 bits.
+  if (L <= kSigLabels) {
+    std::vector<unsigned long long> sig(std::max(g->n, 1u), 0ull);
+    for (uint32_t v = 0; v < g->n; ++v) {
+      const uint32_t* lo = loff.data() + (size_t)v * (L + 1);
+      unsigned long long x = (unsigned long long)(offsets[v + 1] - offsets[v]) << 32;
+      for (uint32_t l = 0; l < L; ++l)
+        x |= (unsigned long long)std::min<uint32_t>(15u, lo[l + 1] - lo[l]) << (4 * l);
+      sig[v] = x;
+    }
+    if (hipMalloc(&g->sig, sig.size() * 8) != hipSuccess) return fail(GPM_ENOMEM, "signatures");
+    GPM_HIP(hipMemcpyAsync(g->sig, sig.data(), sig.size() * 8, hipMemcpyHostToDevice, g->stream));
+    GPM_HIP(hipStreamSynchronize(g->stream));
   }
   g->h_lvoff.assign(L + 1, 0);
   for (uint32_t v = 0; v < g->n; ++v) g->h_lvoff[labels[v] + 1]++;

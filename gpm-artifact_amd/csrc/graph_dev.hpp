@@ -11,6 +11,8 @@
 #include "device_common.hpp"
 #include "tiers.hpp"
 
+constexpr uint32_t kSigLabels = 8;
+
 struct gpm_graph {
   int device = 0;
   uint32_t n = 0;
@@ -28,6 +30,10 @@ struct gpm_graph {
   uint32_t* labels = nullptr;
   uint32_t* ladj = nullptr;
   uint32_t* loff = nullptr;
+  // Label alphabets of at most kSigLabels labels: sig[v] = degree << 32 |
+  // per label l, min(15, neighbours of label l) in bits 4l..4l+3 -- the
+  // matcher's degree and neighbour-label filter in one load.
+  unsigned long long* sig = nullptr;
   // Vertices grouped by label: lverts[h_lvoff[l] .. h_lvoff[l+1]) have label l.
   uint32_t* lverts = nullptr;
   std::vector<uint32_t> h_lvoff;
@@ -71,6 +77,11 @@ struct gpm_graph {
   uint32_t* toff = nullptr;
   uint32_t* tadj = nullptr;
   std::vector<uint32_t> h_rank;
+  // SYMMETRIC graphs: block mask of the counting formulation's prefilter,
+  // bmask[v] bit b set when v has a neighbour w with (w >> bmask_shift) == b,
+  // built on first use.
+  unsigned long long* bmask = nullptr;
+  uint32_t bmask_shift = 0;
 };
 
 namespace gpm {

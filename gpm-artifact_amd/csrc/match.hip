@@ -87,7 +87,15 @@ struct DevGraph {
   const uint32_t* ladj;
   const uint32_t* loff;
   uint32_t L;
+  // Packed degree + neighbour-label counts (label alphabets <= kSigLabels),
+  // else nullptr.
+  const unsigned long long* sig;
 };
+
+// Degree and neighbour-label filter of a candidate x for depth d: one load of
+// x's packed signature when the graph has one, else the degree and the
+// label-slice bounds of every needed label.
+__device__ __forceinline__ bool nlf_ok(const DevGraph& g, const QPlan& p, uint32_t d, uint32_t x);
 
 __device__ __forceinline__ void label_slice(const DevGraph& g, uint32_t w, uint32_t l,
                                             uint32_t* lo, uint32_t* len) {
@@ -136,6 +144,12 @@ struct WaveState {
   uint32_t sl[kMaxQ];    // slice length
   uint32_t anc[kMaxQ];   // anchor depth (its slice is the one streamed)
   unsigned long long mask[kMaxQ];
+  // Slices of depth d's other backward neighbours (the ones an entry of the
+  // anchor slice is probed against), shortest first: fixed while depth d is
+  // streamed, so they are looked up once per DFS node, not once per window.
+  uint32_t np[kMaxQ];
+  uint32_t plo[kMaxQ][kMaxQ - 1];
+  uint32_t plen[kMaxQ][kMaxQ - 1];
 };
 
 // Sets up depth d: picks the backward neighbour with the shortest label slice.
@@ -158,31 +172,51 @@ __device__ __forceinline__ void init_depth(const DevGraph& g, const QPlan& p, Wa
   s.anc[d] = bdep;
   s.cur[d] = 0;
   s.mask[d] = 0ull;
+  uint32_t np = 0;
+  for (uint32_t bm = p.bn[d] & ~(1u << bdep); bm; bm &= bm - 1) {
+    uint32_t lo, len;
+    label_slice(g, s.emb[__builtin_ctz(bm)], p.label[d], &lo, &len);
+    uint32_t at = np++;  // insertion sort by length
+    while (at > 0 && s.plen[d][at - 1] > len) {
+      s.plen[d][at] = s.plen[d][at - 1];
+      s.plo[d][at] = s.plo[d][at - 1];
+      --at;
+    }
+    s.plen[d][at] = len;
+    s.plo[d][at] = lo;
+  }
+  s.np[d] = np;
 }
 
-// Does slice entry x pass the filters of depth d?
-__device__ __forceinline__ bool passes(const DevGraph& g, const QPlan& p, const WaveState& s,
-                                       uint32_t d, uint32_t x) {
+__device__ __forceinline__ bool nlf_ok(const DevGraph& g, const QPlan& p, uint32_t d, uint32_t x) {
+  if (g.sig) {
+    const unsigned long long sg = g.sig[x];
+    if ((uint32_t)(sg >> 32) < p.qdeg[d]) return false;
+    for (uint32_t k = 0; k < p.nlf_n[d]; ++k)
+      if (((uint32_t)(sg >> (4 * p.nlf_label[d][k])) & 15u) < p.nlf_count[d][k]) return false;
+    return true;
+  }
   if (g.off[x + 1] - g.off[x] < p.qdeg[d]) return false;
   for (uint32_t k = 0; k < p.nlf_n[d]; ++k) {
     uint32_t lo, len;
     label_slice(g, x, p.nlf_label[d][k], &lo, &len);
     if (len < p.nlf_count[d][k]) return false;
   }
+  return true;
+}
+
+// Does slice entry x pass the filters of depth d?
+__device__ __forceinline__ bool passes(const DevGraph& g, const QPlan& p, const WaveState& s,
+                                       uint32_t d, uint32_t x) {
   uint32_t same = p.same[d];
   while (same) {
     const uint32_t j = __builtin_ctz(same);
     same &= same - 1;
     if (s.emb[j] == x) return false;
   }
-  uint32_t bn = p.bn[d] & ~(1u << s.anc[d]);
-  while (bn) {
-    const uint32_t b = __builtin_ctz(bn);
-    bn &= bn - 1;
-    uint32_t lo, len;
-    label_slice(g, s.emb[b], p.label[d], &lo, &len);
-    if (!sorted_contains(g.ladj + lo, len, x)) return false;
-  }
+  if (!nlf_ok(g, p, d, x)) return false;
+  for (uint32_t k = 0; k < s.np[d]; ++k)
+    if (!sorted_contains(g.ladj + s.plo[d][k], s.plen[d][k], x)) return false;
   return true;
 }
 
@@ -192,20 +226,17 @@ __device__ __forceinline__ bool passes(const DevGraph& g, const QPlan& p, const 
 // `full`, all but the streamed anchor otherwise.
 __device__ __forceinline__ bool passes_tail(const DevGraph& g, const QPlan& p, const WaveState& s,
                                             uint32_t d, uint32_t x, bool full) {
-  if (g.off[x + 1] - g.off[x] < p.qdeg[d]) return false;
-  for (uint32_t k = 0; k < p.nlf_n[d]; ++k) {
-    uint32_t lo, len;
-    label_slice(g, x, p.nlf_label[d][k], &lo, &len);
-    if (len < p.nlf_count[d][k]) return false;
-  }
   for (uint32_t same = p.same[d] & ((1u << p.tail) - 1u); same; same &= same - 1)
     if (s.emb[__builtin_ctz(same)] == x) return false;
-  uint32_t bn = full ? p.bn[d] : (p.bn[d] & ~(1u << s.anc[d]));
-  while (bn) {
-    const uint32_t b = __builtin_ctz(bn);
-    bn &= bn - 1;
+  if (!nlf_ok(g, p, d, x)) return false;
+  if (!full) {
+    for (uint32_t k = 0; k < s.np[d]; ++k)
+      if (!sorted_contains(g.ladj + s.plo[d][k], s.plen[d][k], x)) return false;
+    return true;
+  }
+  for (uint32_t bn = p.bn[d]; bn; bn &= bn - 1) {
     uint32_t lo, len;
-    label_slice(g, s.emb[b], p.label[d], &lo, &len);
+    label_slice(g, s.emb[__builtin_ctz(bn)], p.label[d], &lo, &len);
     if (!sorted_contains(g.ladj + lo, len, x)) return false;
   }
   return true;
@@ -892,7 +923,7 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
   uint32_t* d_lvoff = reinterpret_cast<uint32_t*>(mp);
   b->n_queries = n_slots;
   b->nq = nq0;
-  b->dg = DevGraph{g->off, g->ladj, g->loff, g->n_labels};
+  b->dg = DevGraph{g->off, g->ladj, g->loff, g->n_labels, g->sig};
   b->grid = (unsigned)device_cus(g) * 8;
   if (nq0 == 0) return GPM_OK;
   int st;

@@ -21,9 +21,9 @@ GPM_OK = 0
 CSR_SYMMETRIZE, CSR_DEDUP, CSR_DROP_SELF, CSR_SORT = 0x1, 0x2, 0x4, 0x8
 CSR_CLEAN = CSR_SYMMETRIZE | CSR_DEDUP | CSR_DROP_SELF | CSR_SORT
 GRAPH_ORIENTED, GRAPH_SYMMETRIC = 0x1, 0x2
-SCHED_DEGREE, SCHED_LEARNED = 0, 1
+SCHED_DEGREE, SCHED_LEARNED, SCHED_DEGREE_SPLIT = 0, 1, 2
 TC_EDGE, TC_VERTEX = 0, 1
-FORM_G2MINER, FORM_PATTERN = 0, 1
+FORM_G2MINER, FORM_PATTERN, FORM_BLOCK_PREFILTER = 0, 1, 2
 PATTERNS = {"P1": 1, "P2": 2, "P3": 3, "P4": 4, "P5": 5, "P6": 6, "P7": 7,
             "P8": 8, "Motif4": 9}
 
@@ -155,6 +155,41 @@ def load_labeled(path: str):
     adj = _take(pa, m.value)
     lab = _take(pl, n.value)
     return HostCSR(off, adj), lab, nl.value
+
+
+def load_ceci_bin(degree_path: str, adj_path: str, label_path: Optional[str] = None):
+    """sub_ceci compressed graph -> (HostCSR with sorted lists, labels or None, n_labels)."""
+    n, po, pa, m, pl, nl = C.c_uint32(), _u32p(), _u32p(), C.c_uint64(), _u32p(), C.c_uint32()
+    _check(lib().gpm_load_ceci_bin(degree_path.encode(), adj_path.encode(),
+                                   label_path.encode() if label_path else None, C.byref(n),
+                                   C.byref(po), C.byref(pa), C.byref(m),
+                                   C.byref(pl) if label_path else None,
+                                   C.byref(nl) if label_path else None))
+    g = HostCSR(_take(po, n.value + 1), _take(pa, m.value))
+    lab = _take(pl, n.value) if label_path else None
+    return g, lab, nl.value
+
+
+def store_ceci_bin(degree_path: str, adj_path: str, g: HostCSR, labels=None,
+                   label_path: Optional[str] = None) -> None:
+    lab = _u32(labels) if labels is not None else None
+    _check(lib().gpm_store_ceci_bin(degree_path.encode(), adj_path.encode(),
+                                    label_path.encode() if label_path else None,
+                                    C.c_uint32(g.n), _ptr(_u32(g.offsets)), _ptr(_u32(g.adj)),
+                                    _ptr(lab) if lab is not None else None))
+
+
+def load_gbin(directory: str):
+    """GraphMiner binary directory -> (HostCSR in file order, max_degree)."""
+    n, po, pa, m, md = C.c_uint32(), _u32p(), _u32p(), C.c_uint64(), C.c_uint32()
+    _check(lib().gpm_load_gbin(directory.encode(), C.byref(n), C.byref(po), C.byref(pa),
+                               C.byref(m), C.byref(md)))
+    return HostCSR(_take(po, n.value + 1), _take(pa, m.value)), md.value
+
+
+def store_gbin(directory: str, g: HostCSR) -> None:
+    _check(lib().gpm_store_gbin(directory.encode(), C.c_uint32(g.n), _ptr(_u32(g.offsets)),
+                                _ptr(_u32(g.adj))))
 
 
 def gen_er(n: int, m: int, seed: int):
@@ -334,6 +369,43 @@ class DeviceGraph:
         _check(lib().gpm_match_batch(self._h, arr, C.c_int(len(queries)),
                                      C.byref(t), _ptr(out, C.c_uint64)))
         return out
+
+
+SETOP_ISECT_COUNT, SETOP_ISECT, SETOP_DIFF_COUNT, SETOP_DIFF, SETOP_FUSE = 0, 1, 2, 3, 4
+SETV_BINARY, SETV_TWO_PHASE, SETV_MERGE, SETV_HASH, SETV_BITMAP = 0, 1, 2, 3, 4
+
+
+def setops(op: int, variant: int, pairs, ub=None):
+    """gpm_setops over a list of (A, B) pairs of strictly increasing uint32
+    lists.  Returns counts (n, or n x 3 for FUSE) and, for ISECT / DIFF, the
+    lists; for FUSE the triples (A \\ B, A & B, B \\ A)."""
+    As = [_u32(a) for a, _ in pairs]
+    Bs = [_u32(b) for _, b in pairs]
+    a_off = np.zeros(len(pairs) + 1, dtype=np.uint64)
+    b_off = np.zeros(len(pairs) + 1, dtype=np.uint64)
+    a_off[1:] = np.cumsum([len(a) for a in As]) if pairs else []
+    b_off[1:] = np.cumsum([len(b) for b in Bs]) if pairs else []
+    A = _u32(np.concatenate(As)) if As else _u32([])
+    B = _u32(np.concatenate(Bs)) if Bs else _u32([])
+    fuse = op == SETOP_FUSE
+    counts = np.zeros(len(pairs) * (3 if fuse else 1), dtype=np.uint64)
+    oa = np.zeros(max(len(A), 1), dtype=np.uint32)
+    oab = np.zeros(max(len(A), 1), dtype=np.uint32)
+    ob = np.zeros(max(len(B), 1), dtype=np.uint32)
+    ub_arr = _u32(ub) if ub is not None else None
+    ubp = _ptr(ub_arr) if ub_arr is not None else None
+    _check(lib().gpm_setops(C.c_int(op), C.c_int(variant), C.c_uint32(len(pairs)),
+                            _ptr(a_off, C.c_uint64), _ptr(A), _ptr(b_off, C.c_uint64), _ptr(B),
+                            ubp, _ptr(counts, C.c_uint64), _ptr(oa), _ptr(oab), _ptr(ob)))
+    if fuse:
+        c = counts.reshape(-1, 3)
+        lists = [(oa[int(a_off[i]):int(a_off[i]) + int(c[i, 0])],
+                  oab[int(a_off[i]):int(a_off[i]) + int(c[i, 1])],
+                  ob[int(b_off[i]):int(b_off[i]) + int(c[i, 2])]) for i in range(len(pairs))]
+        return c, lists
+    if op in (SETOP_ISECT, SETOP_DIFF):
+        return counts, [oa[int(a_off[i]):int(a_off[i]) + int(counts[i])] for i in range(len(pairs))]
+    return counts, None
 
 
 def clique_tier_caps():

@@ -121,32 +121,33 @@ def random_queries(count: int, seed: int, min_size: int = 4, max_size: int = 10,
 
 def dense_queries(g: G.HostCSR, labels: np.ndarray, count: int, seed: int,
                   min_size: int = 4, max_size: int = 10, min_avg_degree: float = 3.0,
-                  max_degree: int = 1 << 30):
+                  max_degree: int = 1 << 30, tries: int = 64):
     """Dense connected queries cut out of g (the shape of the reference's
-    query_dense sets: average degree >= 3 where the size allows it).
+    query_dense sets: average degree about 3 or more).
 
-    A size is drawn uniformly in [min_size, max_size] and a start vertex of
-    degree in [2, max_degree]; the query grows by the data neighbour of its
-    members that has the most edges into the members so far (ties broken at
-    random), and keeps every data edge among its members and their labels.
-    A draw whose average degree stays below min(min_avg_degree, size - 1)
-    is discarded.  Every query has at least one embedding (its own vertices).
-    Deterministic for a given seed.
+    Per query a size is drawn uniformly in [min_size, max_size]; then start
+    vertices of degree in [2, max_degree] are drawn until one grows a dense
+    enough query of that size: the query grows by the data neighbour of its
+    members (degree <= max_degree) with the most edges into the members so
+    far (ties broken at random), and keeps every data edge among its members
+    and their labels.  It is accepted when its average degree reaches
+    min(min_avg_degree, size - 1); after `tries` failed starts the bar is
+    lowered by 0.5.  Every query has at least one embedding (its own
+    vertices).  Deterministic for a given seed.
     """
     rng = np.random.default_rng(seed)
     deg = g.degrees()
     ok = np.nonzero((deg >= 2) & (deg <= max_degree))[0]
-    out = []
-    while len(out) < count:
-        size = int(rng.integers(min_size, max_size + 1))
-        start = int(ok[rng.integers(0, len(ok))])
+
+    def grow(start, size):
         members = [start]
         inside = {start}
         links = {}  # candidate -> edges into the members
-        for w in g.adj[g.offsets[start]:g.offsets[start + 1]]:
-            w = int(w)
-            if deg[w] <= max_degree:
-                links[w] = links.get(w, 0) + 1
+        for b in members:
+            for w in g.adj[g.offsets[b]:g.offsets[b + 1]]:
+                w = int(w)
+                if deg[w] <= max_degree:
+                    links[w] = links.get(w, 0) + 1
         while len(members) < size and links:
             best = max(links.values())
             cands = sorted(v for v, c in links.items() if c == best)
@@ -158,17 +159,32 @@ def dense_queries(g: G.HostCSR, labels: np.ndarray, count: int, seed: int,
                 w = int(w)
                 if w not in inside and deg[w] <= max_degree:
                     links[w] = links.get(w, 0) + 1
-        if len(members) < size:
-            continue
-        idx = {v: i for i, v in enumerate(members)}
-        edges = []
-        for v in members:
-            for w in g.adj[g.offsets[v]:g.offsets[v + 1]]:
-                w = int(w)
-                if w in idx and v < w:
-                    edges.append((idx[v], idx[w]))
-        if 2 * len(edges) < min(min_avg_degree, size - 1) * size:
-            continue
+        return members
+
+    out = []
+    while len(out) < count:
+        size = int(rng.integers(min_size, max_size + 1))
+        bar = min(min_avg_degree, size - 1)
+        while True:
+            found = None
+            for _ in range(tries):
+                members = grow(int(ok[rng.integers(0, len(ok))]), size)
+                if len(members) < size:
+                    continue
+                idx = {v: i for i, v in enumerate(members)}
+                edges = []
+                for v in members:
+                    for w in g.adj[g.offsets[v]:g.offsets[v + 1]]:
+                        w = int(w)
+                        if w in idx and v < w:
+                            edges.append((idx[v], idx[w]))
+                if 2 * len(edges) >= bar * size:
+                    found = (members, edges)
+                    break
+            if found:
+                break
+            bar -= 0.5
+        members, edges = found
         e = np.asarray(sorted(edges), dtype=np.uint32).reshape(-1, 2)
         out.append((size, e, labels[np.asarray(members)].astype(np.uint32)))
     return out

@@ -103,6 +103,35 @@ int gpm_load_labeled(const char* path, uint32_t* n, uint32_t** offsets, uint32_t
                      uint64_t* m, uint32_t** labels, uint32_t* n_labels);
 
 /*
+ * sub_ceci compressed graph (sub_ceci/graph/graph.cpp:184-304): a degree
+ * file (int32 4, uint32 |V|, uint32 |E|, uint32 degree[|V|]), an adjacency
+ * file (the |V| neighbour lists back to back, 2|E| uint32 ids) and, when
+ * label_path is not NULL, a label file (uint32 label[|V|]).  Lists are sorted
+ * on load, as in the reference; *n_labels = max label + 1.  Files whose sizes
+ * disagree with the header, or ids >= |V|, are rejected with GPM_EIO.
+ */
+int gpm_load_ceci_bin(const char* degree_path, const char* adj_path, const char* label_path,
+                      uint32_t* n, uint32_t** offsets, uint32_t** adj, uint64_t* m,
+                      uint32_t** labels, uint32_t* n_labels);
+/* Writes the same three files (Graph::storeComparessedGraph,
+ * sub_ceci/graph/graph.cpp:306-366); label_path may be NULL. */
+int gpm_store_ceci_bin(const char* degree_path, const char* adj_path, const char* label_path,
+                       uint32_t n, const uint32_t* offsets, const uint32_t* adj,
+                       const uint32_t* labels);
+
+/*
+ * GraphMiner binary directory (the output of the reference's converter,
+ * scripts/gbin_reader/greader.h:129-272): graph.vertex.bin (int64 row
+ * pointer, |V|+1), graph.edge.bin (int32 ids, nnz), graph.meta.txt (|V|,
+ * nnz, 4, max degree).  Lists are returned in file order.  Inconsistent
+ * files are rejected with GPM_EIO.
+ */
+int gpm_load_gbin(const char* dir, uint32_t* n, uint32_t** offsets, uint32_t** adj,
+                  uint64_t* m, uint32_t* max_degree);
+/* Writes a CSR as a GraphMiner binary directory (the directory must exist). */
+int gpm_store_gbin(const char* dir, uint32_t n, const uint32_t* offsets, const uint32_t* adj);
+
+/*
  * Seeded Erdos-Renyi G(n, M): M distinct undirected edges, no self loops,
  * emitted once each as (min, max).  No counterpart in the reference (which
  * only reads files); used for the triangle-counting workload.
@@ -166,6 +195,10 @@ int gpm_last_kernel_breakdown(const gpm_graph* g, double* ms, const char** names
 enum gpm_sched_policy {
   GPM_SCHED_DEGREE = 0,  /* plain degree-based scheduler (reference style)  */
   GPM_SCHED_LEARNED = 1, /* cost-model order + chunking from per-root costs */
+  GPM_SCHED_DEGREE_SPLIT = 2, /* k-clique only: LEARNED's ordering, block
+                                 promotion and heavy-root splitting driven by
+                                 the degree estimate d+(v)^2 instead of a
+                                 model (root_cost unused)                   */
 };
 
 typedef struct gpm_sched {
@@ -244,6 +277,9 @@ enum gpm_pattern_id {
 enum gpm_formulation {
   GPM_FORM_G2MINER = 0, /* direct enumeration (G2MinerSolver)              */
   GPM_FORM_PATTERN = 1, /* frequency x count - correction (PatternSolver)   */
+  GPM_FORM_BLOCK_PREFILTER = 2, /* PatternSolver with the BLOCK_OPT block-
+                                   mask prefilter for P4, P5, P6
+                                   (PatternSolver.cuh:109-151)              */
 };
 
 /*
@@ -251,11 +287,51 @@ enum gpm_formulation {
  * (src/G2MinerSolver.cuh:18-175) and PatternSolver (src/PatternSolver.cuh:
  * 17-264).  out[0..5]: for P1..P8 out[0] is the pattern count; for MOTIF4
  * out = induced {3-star, 4-path, tailed-triangle, 4-cycle, diamond,
- * 4-clique}.  *scalar receives what the matching reference solver returns
- * (G2Miner: counters[0]; PatternSolver: h[1] or h[1]-h[2], modulo 2^64).
+ * 4-clique}.  Under the counting formulations, P2, P3, P7 and P8 also report
+ * the frequency-weighted sum in out[1] and the correction in out[2]
+ * (out[0] = out[1] - out[2]), and P4 (prefilter only), P5 and P6 the pair
+ * count in out[1] (= out[0]).  *scalar receives what the matching reference
+ * solver returns (G2Miner: counters[0]; PatternSolver: h[1] or h[1]-h[2],
+ * modulo 2^64).
  */
 int gpm_pattern(gpm_graph* g, int pattern_id, int formulation,
                 uint64_t out[6], uint64_t* scalar);
+
+/* ------------------------------------------------------ set primitives */
+
+enum gpm_setop_kind {
+  GPM_SETOP_ISECT_COUNT = 0, /* |{x in A & B : x < ub}|  (intersect_num)     */
+  GPM_SETOP_ISECT = 1,       /* that list, sorted         (intersect)         */
+  GPM_SETOP_DIFF_COUNT = 2,  /* |{x in A \ B : x < ub}| (difference_num)      */
+  GPM_SETOP_DIFF = 3,        /* that list, sorted         (difference_set)    */
+  GPM_SETOP_FUSE = 4,        /* A \ B, A & B and B \ A in one pass (fuse_set) */
+};
+enum gpm_set_variant {
+  GPM_SETV_BINARY = 0,    /* per-lane binary search of B (search.cuh:20-35)   */
+  GPM_SETV_TWO_PHASE = 1, /* 64 LDS pivots, then a 1/64 slice of B
+                             (binary_search_2phase, search.cuh:54-87)        */
+  GPM_SETV_MERGE = 2,     /* merge path: 64 diagonal segments, one per lane  */
+  GPM_SETV_HASH = 3,      /* B in an LDS hash table (|B| <= 512, else binary) */
+  GPM_SETV_BITMAP = 4,    /* B in an LDS bitmap over [B[0], B[|B|-1]]
+                             (span <= 32768, else binary)                    */
+};
+
+/*
+ * Batched sorted-set operations, one wave per pair, replacing the primitives
+ * of src/graph/set_intersection.cuh:6-304 and set_difference.cuh:6-325.
+ * Pair i is A_i = a[a_off[i] .. a_off[i+1]) and B_i = b[b_off[i] ..
+ * b_off[i+1]), each strictly increasing.  A is the iterated list (A \ B is
+ * never swapped); ub (NULL = none; ignored by FUSE) keeps only x < ub[i].
+ * counts[i] receives the size (FUSE: counts[3i .. 3i+2] = |A\B|, |A&B|,
+ * |B\A|).  ISECT, DIFF and FUSE write list i at out_a + a_off[i] (FUSE:
+ * A \ B there, A & B at out_ab + a_off[i], B \ A at out_b + b_off[i]).
+ * Host arrays; synchronous.  No single counterpart in the reference: its
+ * primitives are device functions called inside the mining kernels.
+ */
+int gpm_setops(int op, int variant, uint32_t n_pairs, const uint64_t* a_off,
+               const uint32_t* a, const uint64_t* b_off, const uint32_t* b,
+               const uint32_t* ub, uint64_t* counts, uint32_t* out_a,
+               uint32_t* out_ab, uint32_t* out_b);
 
 /* Query graph for the labeled matcher (<= 16 vertices, connected). */
 typedef struct gpm_query {

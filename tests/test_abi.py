@@ -32,3 +32,4 @@ def test_error_path_without_gpu(gpm):
     st = lib.gpm_csr_orient(ctypes.c_uint32(0), None, None, None, None, None)
     assert st == 1
     assert b"null" in lib.gpm_last_error()
+

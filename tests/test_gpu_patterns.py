@@ -50,7 +50,41 @@ def test_patterns_direct(gpm, orc, graph_cases, pid):
             out, scalar = dg.pattern(f"P{pid}", gpm.FORM_G2MINER)
             out2, scalar2 = dg.pattern(f"P{pid}", gpm.FORM_PATTERN)
         assert out[0] == want and scalar == want, (name, pid)
-        assert scalar2 == want, (name, pid)
+        assert scalar2 == want and out2[0] == want, (name, pid)
+        if pid in (2, 3, 7, 8):
+            # frequency x count - correction: both sums reported
+            assert out2[1] - out2[2] == want, (name, pid)
+        if pid in (5, 6):
+            assert out2[1] == want, (name, pid)
+
+
+@pytest.mark.parametrize("pid", [2, 3, 4, 5, 6, 7, 8])
+def test_patterns_counting_formulation_larger(gpm, orc, pid):
+    """The counting formulation (frequency x count - correction for P2, P3,
+    P7, P8; pair counts from the shorter list for P4-P6, with and without the
+    block prefilter) against the oracle on R-MAT graphs with hubs, and the
+    direct kernels on the same graph."""
+    for g in (gpm.rmat_graph(10, 8, 40 + pid), graphs.disjoint_union(graphs.wheel(300),
+                                                                      graphs.er(200, 1500, pid))):
+        want = orc.pattern(g.offsets, g.adj, pid)
+        with gpm.DeviceGraph(g, oriented=False) as dg:
+            got = dg.pattern(f"P{pid}", gpm.FORM_PATTERN)
+            pre = dg.pattern(f"P{pid}", gpm.FORM_BLOCK_PREFILTER)
+            direct = dg.pattern(f"P{pid}", gpm.FORM_G2MINER)
+        assert got[1] == want and got[0][0] == want, pid
+        assert pre[1] == want and pre[0][0] == want, pid
+        assert direct[1] == want, pid
+
+
+def test_block_prefilter_edge_cases(gpm, orc):
+    """Block masks on graphs whose ids span one block, exactly 64 blocks and
+    more: P4-P6 under the prefilter equal the oracle."""
+    for g in (graphs.complete(5), graphs.wheel(64), graphs.wheel(65), gpm.rmat_graph(8, 8, 9),
+              graphs.from_edges(3, [(0, 1)])):
+        with gpm.DeviceGraph(g, oriented=False) as dg:
+            for pid in (4, 5, 6):
+                assert dg.pattern(f"P{pid}", gpm.FORM_BLOCK_PREFILTER)[1] == \
+                    orc.pattern(g.offsets, g.adj, pid), pid
 
 
 def test_motif4_identities_larger(gpm, orc):

@@ -133,6 +133,9 @@ def test_learned_split_counts_identical(gpm, k):
         for cost in (deg ** 3, deg, rng.random(d.n).astype(np.float32) * 1e6,
                      np.zeros(d.n, np.float32)):
             assert dag.clique(k, policy=gpm.SCHED_LEARNED, cost=cost) == plain
+        # the degree-estimate split (same splitting, cost d+^2, no model)
+        assert dag.clique(k, policy=gpm.SCHED_DEGREE_SPLIT) == plain
+        assert dag.clique_all(policy=gpm.SCHED_DEGREE_SPLIT)[k - 3] == plain
 
 
 @pytest.mark.parametrize("n", [40, 300, 700, 1100])

@@ -138,3 +138,87 @@ def test_load_labeled_rejects(gpm, tmp_path, text):
         gpm.load_labeled(str(p))
     with pytest.raises(gpm.GpmError):
         gpm.load_labeled(str(tmp_path / "missing.graph"))
+
+
+def test_ceci_bin_byte_layout(gpm, tmp_path):
+    """The compressed format as sub_ceci/graph/graph.cpp:184-304 reads it:
+    int32 4, |V|, |E|, degrees; lists back to back (sorted on load); labels."""
+    import struct
+    # path 0-1-2 plus triangle 2-3-4: lists written unsorted on purpose
+    lists = [[1], [2, 0], [4, 3, 1], [2, 4], [3, 2]]
+    deg = str(tmp_path / "g_deg.bin")
+    adj = str(tmp_path / "g_adj.bin")
+    lab = str(tmp_path / "g_label.bin")
+    with open(deg, "wb") as f:
+        f.write(struct.pack("<iII", 4, 5, 5) + struct.pack("<5I", *[len(l) for l in lists]))
+    with open(adj, "wb") as f:
+        for l in lists:
+            f.write(struct.pack(f"<{len(l)}I", *l))
+    with open(lab, "wb") as f:
+        f.write(struct.pack("<5I", 3, 0, 7, 0, 3))
+    g, labels, nl = gpm.load_ceci_bin(deg, adj, lab)
+    assert g.offsets.tolist() == [0, 1, 3, 6, 8, 10]
+    assert g.adj.tolist() == [1, 0, 2, 1, 3, 4, 2, 4, 2, 3]
+    assert labels.tolist() == [3, 0, 7, 0, 3] and nl == 8
+    # Writer round trip gives the same bytes for sorted input.
+    gpm.store_ceci_bin(str(tmp_path / "r_deg.bin"), str(tmp_path / "r_adj.bin"), g, labels,
+                       str(tmp_path / "r_label.bin"))
+    assert open(str(tmp_path / "r_deg.bin"), "rb").read() == open(deg, "rb").read()
+    g2, l2, _ = gpm.load_ceci_bin(str(tmp_path / "r_deg.bin"), str(tmp_path / "r_adj.bin"),
+                                  str(tmp_path / "r_label.bin"))
+    assert np.array_equal(g2.adj, g.adj) and np.array_equal(l2, labels)
+
+
+def test_ceci_bin_roundtrip_hprd_and_rejects(gpm, tmp_path):
+    from test_ceci import load_golden
+    g, labels, _, _, _ = load_golden(gpm)
+    d, a, l = (str(tmp_path / x) for x in ("d.bin", "a.bin", "l.bin"))
+    gpm.store_ceci_bin(d, a, g, labels, l)
+    g2, l2, nl = gpm.load_ceci_bin(d, a, l)
+    assert np.array_equal(g2.offsets, g.offsets) and np.array_equal(g2.adj, g.adj)
+    assert np.array_equal(l2, labels) and nl == int(labels.max()) + 1
+    g3, l3, _ = gpm.load_ceci_bin(d, a)  # no label file
+    assert l3 is None and np.array_equal(g3.adj, g.adj)
+    # A truncated adjacency file and a wrong int size are rejected.
+    raw = open(a, "rb").read()
+    open(a, "wb").write(raw[:-4])
+    with pytest.raises(gpm.GpmError):
+        gpm.load_ceci_bin(d, a, l)
+    raw = bytearray(open(d, "rb").read())
+    raw[0] = 8
+    open(d, "wb").write(bytes(raw))
+    with pytest.raises(gpm.GpmError):
+        gpm.load_ceci_bin(d, a, l)
+
+
+def test_gbin_byte_layout_and_roundtrip(gpm, tmp_path):
+    """GraphMiner binary (scripts/gbin_reader/greader.h:129-272): int64 row
+    pointers, int32 ids in input order, meta |V| nnz 4 max_degree."""
+    import struct
+    d = tmp_path / "gm"
+    d.mkdir()
+    rows = [[2, 1], [], [0], [2, 0, 1]]
+    rp = [0]
+    for r in rows:
+        rp.append(rp[-1] + len(r))
+    (d / "graph.vertex.bin").write_bytes(struct.pack(f"<{len(rp)}q", *rp))
+    flat = [x for r in rows for x in r]
+    (d / "graph.edge.bin").write_bytes(struct.pack(f"<{len(flat)}i", *flat))
+    (d / "graph.meta.txt").write_text("4\n6\n4\n3\n")
+    g, maxd = gpm.load_gbin(str(d))
+    assert g.offsets.tolist() == rp and g.adj.tolist() == flat and maxd == 3
+    e = tmp_path / "out"
+    e.mkdir()
+    gpm.store_gbin(str(e), g)
+    for name in ("graph.vertex.bin", "graph.edge.bin", "graph.meta.txt"):
+        assert (e / name).read_bytes() == (d / name).read_bytes(), name
+    big = gpm.rmat_graph(10, 8, 5)
+    gpm.store_gbin(str(e), big)
+    g2, maxd2 = gpm.load_gbin(str(e))
+    assert np.array_equal(g2.offsets, big.offsets) and np.array_equal(g2.adj, big.adj)
+    assert maxd2 == int(big.degrees().max())
+    (e / "graph.meta.txt").write_text(f"{big.n}\n{big.m}\n4\n{maxd2 + 1}\n")
+    with pytest.raises(gpm.GpmError):  # max degree disagrees with the data
+        gpm.load_gbin(str(e))
+    with pytest.raises(gpm.GpmError):
+        gpm.load_gbin(str(tmp_path / "missing"))
