@@ -15,12 +15,25 @@
 //                LDS capacity probe N+(u) by binary search instead.
 //  Count = sum over oriented edges (u, v) of |N+(u) & N+(v)|.
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "graph_dev.hpp"
 #include "tiers.hpp"
 
 using namespace gpm;
+
+// Scratch of gpm_tc_edge_noalloc: device memory the library never
+// allocates -- each GPU holds its own copy of this array from the moment the
+// code object is loaded until the process ends.  Layout: one uint64 counter
+// at byte 0, then the COO source ids (uint32) of up to kNoAllocMaxEdges
+// oriented edges from byte kNoAllocSrcOffset.  Host code must not use the
+// array's name as a device address: it resolves the per-device address with
+// hipGetSymbolAddress once per device (noalloc_scratch below).
+constexpr uint64_t kNoAllocMaxEdges = 16ull << 20;  // 16,777,216 oriented edges
+constexpr size_t kNoAllocSrcOffset = 256;
+constexpr size_t kNoAllocScratchBytes = kNoAllocSrcOffset + kNoAllocMaxEdges * sizeof(uint32_t);
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 namespace {
 
@@ -207,6 +220,26 @@ __global__ __launch_bounds__(kBlock) void tc_vertex_block_kernel(
   if (lane_id() == 0 && cnt) atomicAdd(out, (unsigned long long)cnt);
 }
 
+// src[e] = u for every e in [off[u], off[u+1]) with e < m; one wave per
+// vertex, grid-stride.  Entries past m are never written, whatever the
+// offsets hold.
+__global__ void noalloc_src_kernel(const uint32_t* __restrict__ off, uint32_t n, uint64_t m,
+                                   uint32_t* __restrict__ src) {
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
+  const uint32_t lane = lane_id();
+  for (uint32_t u = wave; u < n; u += nwaves) {
+    const uint64_t b = off[u], e = min<uint64_t>(off[u + 1], m);
+    for (uint64_t i = b + lane; i < e; i += kWave) src[i] = u;
+  }
+}
+
+// *dst = *src (one thread): hands the scratch counter to the caller's output.
+__global__ void copy_u64_kernel(const unsigned long long* __restrict__ src,
+                                unsigned long long* __restrict__ dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = *src;
+}
+
 __global__ void zero_u32_kernel(unsigned int* p, int count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) p[i] = 0u;
@@ -217,8 +250,8 @@ __global__ void zero_u64_kernel(unsigned long long* p, int count) {
   if (i < count) p[i] = 0ull;
 }
 
-int edge_group_size(const gpm_graph* g) {
-  const double avg = g->n ? (double)g->m / g->n : 0.0;
+int edge_group_size(uint32_t n, uint64_t m) {
+  const double avg = n ? (double)m / n : 0.0;
   if (avg <= 6) return 4;
   if (avg <= 12) return 8;
   if (avg <= 24) return 16;
@@ -233,7 +266,7 @@ int enqueue_tc(gpm_graph* g, int variant, bool mark) {
   const int cus = device_cus(g);
   if (variant == GPM_TC_EDGE) {
     if ((st = zero_counters(g, 1)) != GPM_OK) return st;
-    const int G = edge_group_size(g);
+    const int G = edge_group_size(g->n, g->m);
     const uint64_t groups_wanted = std::max<uint64_t>(g->m, 1);
     const uint64_t blocks = std::min<uint64_t>((groups_wanted * G + kBlock - 1) / kBlock,
                                                (uint64_t)cus * 32);
@@ -301,7 +334,7 @@ int ensure_tc_graph(gpm_graph* g) {
   zp.kernelParams = zargs;
   hipGraphNode_t zn = nullptr;
   GPM_HIP(hipGraphAddKernelNode(&zn, g->tc_graph, nullptr, 0, &zp));
-  const int G = edge_group_size(g);
+  const int G = edge_group_size(g->n, g->m);
   const int cus = device_cus(g);
   const uint64_t blocks = std::max<uint64_t>(
       1, std::min<uint64_t>((std::max<uint64_t>(g->m, 1) * G + kBlock - 1) / kBlock,
@@ -367,9 +400,89 @@ int tc_run(gpm_graph* g, int variant, int reps, uint64_t* out) {
   return GPM_OK;
 }
 
+// Device address of `scratch` on the current device, resolved once per
+// device with that device current and kept; every later call on the device
+// uses the kept address.
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+int noalloc_scratch(void** out) {
+  int dev = 0;
+  GPM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return fail(GPM_EUNSUPPORTED, "device %d", dev);
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(GPM_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                  hipGetErrorString(e));
+    g_noalloc_ptr[dev] = p;
+  }
+  *out = g_noalloc_ptr[dev];
+  return GPM_OK;
+}
+
+int tc_edge_noalloc(const uint32_t* off, const uint32_t* adj, uint32_t n, uint64_t m,
+                    unsigned long long* count, hipStream_t stream) {
+  GPM_CHECK_ARG(off && count, "null pointer");
+  GPM_CHECK_ARG(adj || m == 0, "null adjacency with m = %llu", (unsigned long long)m);
+  GPM_CHECK_ARG(n < 0xFFFFFFFFu, "n = %u too large", n);
+  GPM_CHECK_ARG(m <= kNoAllocMaxEdges,
+                "m = %llu oriented edges exceeds the no-alloc limit of %llu (its COO staging "
+                "lives in a fixed %zu-byte device array)",
+                (unsigned long long)m, (unsigned long long)kNoAllocMaxEdges,
+                kNoAllocScratchBytes);
+  void* base = nullptr;
+  int st;
+  if ((st = noalloc_scratch(&base)) != GPM_OK) return st;
+  unsigned long long* counter = static_cast<unsigned long long*>(base);
+  uint32_t* src = reinterpret_cast<uint32_t*>(static_cast<unsigned char*>(base) + kNoAllocSrcOffset);
+  int dev = 0, cus = 256;
+  GPM_HIP(hipGetDevice(&dev));
+  GPM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  // The counter keeps the previous call's value: clear it with a kernel (no
+  // memset node, so the sequence may be captured into a hipGraph).
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, stream, counter, 1);
+  GPM_HIP(hipGetLastError());
+  if (m) {
+    const uint64_t sblocks = std::max<uint64_t>(
+        1, std::min<uint64_t>(((uint64_t)n * kWave + kBlock - 1) / kBlock, (uint64_t)cus * 16));
+    hipLaunchKernelGGL(noalloc_src_kernel, dim3((unsigned)sblocks), dim3(kBlock), 0, stream, off,
+                       n, m, src);
+    GPM_HIP(hipGetLastError());
+    const int G = edge_group_size(n, m);
+    const uint64_t blocks = std::max<uint64_t>(
+        1, std::min<uint64_t>((m * G + kBlock - 1) / kBlock, (uint64_t)cus * 32));
+    dim3 grid((unsigned)blocks), block(kBlock);
+    switch (G) {
+      case 4: hipLaunchKernelGGL(tc_edge_kernel<4>, grid, block, 0, stream, off, adj, src, m, counter); break;
+      case 8: hipLaunchKernelGGL(tc_edge_kernel<8>, grid, block, 0, stream, off, adj, src, m, counter); break;
+      case 16: hipLaunchKernelGGL(tc_edge_kernel<16>, grid, block, 0, stream, off, adj, src, m, counter); break;
+      case 32: hipLaunchKernelGGL(tc_edge_kernel<32>, grid, block, 0, stream, off, adj, src, m, counter); break;
+      default: hipLaunchKernelGGL(tc_edge_kernel<64>, grid, block, 0, stream, off, adj, src, m, counter); break;
+    }
+    GPM_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(copy_u64_kernel, dim3(1), dim3(64), 0, stream, counter, count);
+  GPM_HIP(hipGetLastError());
+  return GPM_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int gpm_tc_edge_noalloc(const uint32_t* d_offsets, const uint32_t* d_adj, uint32_t n,
+                        uint64_t m, uint64_t* d_count, void* stream) {
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    return tc_edge_noalloc(d_offsets, d_adj, n, m,
+                           reinterpret_cast<unsigned long long*>(d_count),
+                           static_cast<hipStream_t>(stream));
+  });
+}
 
 int gpm_tc(gpm_graph* g, int variant, uint64_t* out) {
   return ::gpm::guarded([&]() -> int {

@@ -371,6 +371,16 @@ class DeviceGraph:
         return out
 
 
+def tc_edge_noalloc(d_offsets: int, d_adj: int, n: int, m: int, d_count: int,
+                    stream: int = 0) -> None:
+    """gpm_tc_edge_noalloc on device addresses (ints): enqueues the count of
+    the oriented CSR at d_offsets/d_adj into the device uint64 at d_count on
+    `stream` (a hipStream_t as an int) and returns without waiting."""
+    _check(lib().gpm_tc_edge_noalloc(C.c_void_p(d_offsets), C.c_void_p(d_adj),
+                                     C.c_uint32(n), C.c_uint64(m), C.c_void_p(d_count),
+                                     C.c_void_p(stream)))
+
+
 SETOP_ISECT_COUNT, SETOP_ISECT, SETOP_DIFF_COUNT, SETOP_DIFF, SETOP_FUSE = 0, 1, 2, 3, 4
 SETV_BINARY, SETV_TWO_PHASE, SETV_MERGE, SETV_HASH, SETV_BITMAP = 0, 1, 2, 3, 4
 

@@ -235,6 +235,26 @@ int gpm_tc(gpm_graph* g, int variant, uint64_t* out);
 int gpm_tc_repeat(gpm_graph* g, int variant, int reps, uint64_t* out);
 
 /*
+ * Edge-parallel triangle count (the GPM_TC_EDGE kernel of gpm_tc) that
+ * allocates nothing.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory -- code under hipGraph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.
+ *
+ * Inputs are device pointers to an oriented, sorted CSR (d_offsets: n + 1
+ * entries with d_offsets[n] == m; d_adj: m ids < n); the count is written to
+ * the device uint64 *d_count.  The call only enqueues kernels on `stream` (a
+ * hipStream_t; NULL = the null stream) and returns: *d_count holds the count
+ * once the stream has reached that point.  Its staging (the COO source of
+ * every edge and the counter) lives in a fixed device array of
+ * 256 + 4 * 16,777,216 bytes that each GPU keeps until the process ends;
+ * m > 16,777,216 is rejected with GPM_EINVAL.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (two host threads, or two
+ * streams running at once) race and must be serialised by the caller.
+ */
+int gpm_tc_edge_noalloc(const uint32_t* d_offsets, const uint32_t* d_adj, uint32_t n,
+                        uint64_t m, uint64_t* d_count, void* stream);
+
+/*
  * k-clique count (3 <= k <= 5) of an ORIENTED graph, replacing CFSolver
  * (src/CFSolver.cuh:99-157, which supports k = 4, 5).  sched may be NULL.
  */

@@ -33,3 +33,14 @@ def test_error_path_without_gpu(gpm):
     assert st == 1
     assert b"null" in lib.gpm_last_error()
 
+
+def test_noalloc_limit_rejected_without_gpu(gpm):
+    # The edge-count limit of the no-alloc triangle count is checked before
+    # anything touches a device.
+    lib = gpm.lib()
+    st = lib.gpm_tc_edge_noalloc(ctypes.c_void_p(16), ctypes.c_void_p(16), ctypes.c_uint32(10),
+                                 ctypes.c_uint64((16 << 20) + 1), ctypes.c_void_p(16), None)
+    assert st == 1
+    assert b"no-alloc limit" in lib.gpm_last_error()
+    st = lib.gpm_tc_edge_noalloc(None, None, ctypes.c_uint32(0), ctypes.c_uint64(0), None, None)
+    assert st == 1
