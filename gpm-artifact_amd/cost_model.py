@@ -30,6 +30,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def default_path(k: int) -> str:
+    """k-clique root-cost model for k >= 3; k = 0 is the matching root-cost
+    model (features: degree, depth-1 label-slice length, degrees summed over
+    that slice, query size; target: weighted DFS steps, gpm_match_root_work)."""
+    if k == 0:
+        return os.path.join(HERE, "weights", "match_root.npz")
     return os.path.join(HERE, "weights", f"clique_k{k}.npz")
 
 
@@ -57,6 +62,11 @@ class CostModel:
     def save(self, path: str) -> None:
         os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
         np.savez(path, k=np.int64(self.k), **self.params)
+
+    def packed(self) -> np.ndarray:
+        """The parameters as one float32 vector in gpm_mlp_predict's argument
+        order (mean, std, W1, b1, W2, b2, W3, b3): gpm_tuning.root_model."""
+        return np.concatenate([self.params[f].reshape(-1) for f in FIELDS]).astype(np.float32)
 
     def predict_log(self, feat: np.ndarray) -> np.ndarray:
         """log1p(work) per row of feat (n x 4), evaluated on the GPU."""

@@ -281,6 +281,56 @@ int build_label_index(gpm_graph* g, const uint32_t* offsets, const uint32_t* adj
 
 }  // namespace
 
+namespace gpm {
+
+int adopt_device_csr(uint32_t n, uint64_t m, uint32_t* d_off, uint32_t* d_adj,
+                     std::vector<uint32_t>&& h_off, const uint32_t* h_adj, const uint32_t* labels,
+                     uint32_t n_labels, uint32_t flags, hipStream_t stream, gpm_graph** out) {
+  *out = nullptr;
+  int dev = 0;
+  GPM_HIP(hipGetDevice(&dev));
+  gpm_graph* g = new gpm_graph();
+  g->device = dev;
+  g->n = n;
+  g->m = m;
+  g->flags = flags;
+  g->n_labels = labels ? n_labels : 0;
+  g->off = d_off;
+  g->adj = d_adj;
+  g->stream = stream;
+  for (uint32_t v = 0; v < n; ++v) g->max_deg = std::max(g->max_deg, h_off[v + 1] - h_off[v]);
+  g->h_off = std::move(h_off);
+  if (labels) g->h_labels.assign(labels, labels + n);
+  int st = GPM_OK;
+  do {
+    if (hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess) {
+      st = fail(GPM_EHIP, "event creation failed");
+      break;
+    }
+    if (hipMalloc(&g->counters, 64 * sizeof(unsigned long long)) != hipSuccess) {
+      st = fail(GPM_ENOMEM, "device counters");
+      break;
+    }
+    if (labels) {
+      if (hipMalloc(&g->labels, (size_t)std::max(n, 1u) * 4) != hipSuccess) {
+        st = fail(GPM_ENOMEM, "device labels");
+        break;
+      }
+      if ((st = h2d(g, g->labels, labels, (size_t)n * 4)) != GPM_OK) break;
+      if ((st = build_label_index(g, g->h_off.data(), h_adj, labels)) != GPM_OK) break;
+    }
+    if (hipStreamSynchronize(g->stream) != hipSuccess) st = fail(GPM_EHIP, "stream synchronize");
+  } while (0);
+  if (st != GPM_OK) {
+    release(g);  // frees d_off, d_adj and the stream with the rest
+    return st;
+  }
+  *out = g;
+  return GPM_OK;
+}
+
+}  // namespace gpm
+
 extern "C" {
 
 int gpm_graph_create(const uint32_t* offsets, uint32_t n, const uint32_t* adj,

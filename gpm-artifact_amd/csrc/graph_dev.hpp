@@ -108,6 +108,21 @@ int timing_end(gpm_graph* g);
 int device_cus(const gpm_graph* g);
 // Builds g->toff/tadj/h_rank if missing; GPM_EINVAL if the graph has a cycle.
 int ensure_topo(gpm_graph* g);
+// Wraps a CSR already resident on the device (d_off: n + 1 entries, d_adj:
+// m) into a new handle that owns d_off, d_adj and `stream`; h_off is the host
+// copy of the offsets, h_adj the host copy of the adjacency (needed only
+// with labels, for the label index).  On failure everything given is freed.
+int adopt_device_csr(uint32_t n, uint64_t m, uint32_t* d_off, uint32_t* d_adj,
+                     std::vector<uint32_t>&& h_off, const uint32_t* h_adj, const uint32_t* labels,
+                     uint32_t n_labels, uint32_t flags, hipStream_t stream, gpm_graph** out);
+// Floats of a packed MLP: mean[4], std[4], W1[64x4], b1[64], W2[64x64],
+// b2[64], W3[64], b3[1] (the order of gpm_mlp_predict's arguments).
+constexpr size_t kMlpWeights = 4 + 4 + 64 * 4 + 64 + 64 * 64 + 64 + 64 + 1;
+// log1p-domain outputs of the packed MLP at d_packed for n feature rows
+// (4 floats each, raw -- log1p and normalisation happen inside), enqueued on
+// `stream` (mlp.hip).
+int mlp_forward(const float* d_feat, uint32_t n, const float* d_packed, float* d_out, int cus,
+                hipStream_t stream);
 // Releases the cached matching batch of g (match.hip).
 void free_match_cache(gpm_graph* g);
 

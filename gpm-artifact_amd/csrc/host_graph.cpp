@@ -95,7 +95,7 @@ using namespace gpm;
 extern "C" {
 
 const char* gpm_last_error(void) { return g_err.c_str(); }
-int gpm_abi_version(void) { return 1; }
+int gpm_abi_version(void) { return 2; }
 void gpm_free(void* p) { std::free(p); }
 
 int gpm_csr_from_edges(uint32_t n, const uint32_t* src, const uint32_t* dst,

@@ -136,6 +136,38 @@ __global__ __launch_bounds__(kBlock) void roots_kernel(DevGraph g, const uint32_
   }
 }
 
+// Features of the root candidates for the learned root-cost model, one wave
+// per root (plan = blockIdx.y): deg(v), the length of v's depth-1 label
+// slice, the degrees summed over that slice (the 2-hop entries a DFS below v
+// can reach) and the query size.  Rows of absent roots stay as they are.
+__global__ __launch_bounds__(kBlock) void root_features_kernel(
+    DevGraph g, const QPlan* __restrict__ plans, const uint32_t* __restrict__ roots,
+    const unsigned int* __restrict__ rootcnt, float* __restrict__ feat) {
+  const QPlan& p = plans[blockIdx.y];
+  const uint32_t nr = rootcnt[blockIdx.y];
+  const uint32_t lane = lane_id();
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
+  for (uint32_t r = wave; r < nr; r += nwaves) {
+    const uint32_t v = roots[p.root_off + r];
+    uint32_t lo = 0, len = 0;
+    if (p.n >= 2) label_slice(g, v, p.label[1], &lo, &len);
+    uint64_t hop = 0;
+    for (uint32_t i = lane; i < len; i += kWave) {
+      const uint32_t u = g.ladj[lo + i];
+      hop += g.off[u + 1] - g.off[u];
+    }
+    hop = wave_sum_u64(hop);
+    if (lane == 0) {
+      float* f = feat + 4 * (size_t)(p.root_off + r);
+      f[0] = (float)(g.off[v + 1] - g.off[v]);
+      f[1] = (float)len;
+      f[2] = (float)hop;
+      f[3] = (float)p.n;
+    }
+  }
+}
+
 struct WaveState {
   uint32_t emb[kMaxQ];
   uint32_t cur[kMaxQ];   // next slice position to load
@@ -247,6 +279,11 @@ __device__ __forceinline__ bool passes_tail(const DevGraph& g, const QPlan& p, c
 // duration stays bounded whatever the candidates cost.
 constexpr uint64_t kTrialBudget = 1ull << 20;
 
+// Kernel modes: count embeddings; plan trial (weighted steps per plan,
+// budget-limited); root log (weighted steps of every task, one root per
+// task, no budget -- the training data of the learned root-cost model).
+constexpr int kModeCount = 0, kModeTrial = 1, kModeLog = 2;
+
 // Trial cost of one 64-entry window at depth d: the dependent loads one
 // lane makes -- the entry, its degree and NLF counts, and per probed
 // backward neighbour its slice bounds plus a binary search of that slice
@@ -270,7 +307,7 @@ __device__ uint64_t window_cost(const DevGraph& g, const QPlan& p, const WaveSta
 // |A_B| for a subset B is counted while streaming the candidates of B's
 // first member (each x tested against the later members' full filters).
 // Arithmetic is modulo 2^64, exact when the true count fits.
-template <bool kTrial>
+template <int kMode>
 __device__ uint64_t count_tail(const DevGraph& g, const QPlan& p, WaveState& s, uint64_t& steps) {
   const uint32_t lane = lane_id();
   uint64_t prod = 1;
@@ -282,12 +319,12 @@ __device__ uint64_t count_tail(const DevGraph& g, const QPlan& p, WaveState& s, 
       init_depth(g, p, s, d);
       const uint32_t sl = uniform(s.sl[d]), sb = uniform(s.sb[d]);
       uint64_t wcost = 1;
-      if (kTrial) {
+      if (kMode != kModeCount) {
         wcost = window_cost(g, p, s, d, false);
         for (uint32_t m2 = m + 1; m2 < k; ++m2) wcost += window_cost(g, p, s, p.grp_dep[gi][m2], true);
       }
       for (uint32_t cur = 0; cur < sl; cur += kWave) {
-        if (kTrial && steps >= kTrialBudget) break;
+        if (kMode == kModeTrial && steps >= kTrialBudget) break;
         steps += wcost;
         const uint32_t i = cur + lane;
         uint32_t mask = 0;
@@ -321,21 +358,21 @@ __device__ uint64_t count_tail(const DevGraph& g, const QPlan& p, WaveState& s, 
 // Counts the embeddings of plan p rooted at v0 whose depth-1 vertex is one
 // of the entries [lo1, hi1) of the depth-1 slice (all: 0, ~0u): a DFS over
 // depths 1..tail-1, each leaf of which adds the tail count.
-template <bool kTrial>
+template <int kMode>
 __device__ uint64_t count_root(const DevGraph& g, const QPlan& p, WaveState& s, uint32_t v0,
                                uint32_t lo1, uint32_t hi1, uint64_t& steps) {
   const uint32_t lane = lane_id();
   const uint32_t n = p.n;
   if (n == 1) return 1;
   s.emb[0] = v0;
-  if (p.tail == 1) return count_tail<kTrial>(g, p, s, steps);
+  if (p.tail == 1) return count_tail<kMode>(g, p, s, steps);
   uint64_t total = 0;
   init_depth(g, p, s, 1);
   s.cur[1] = lo1;
   s.sl[1] = min(s.sl[1], hi1);
   int d = 1;
   while (d >= 1) {
-    if (kTrial && steps >= kTrialBudget) break;
+    if (kMode == kModeTrial && steps >= kTrialBudget) break;
     const uint32_t dd = (uint32_t)d;
     if (s.mask[dd] == 0ull) {
       const uint32_t cur = s.cur[dd];
@@ -343,7 +380,7 @@ __device__ uint64_t count_root(const DevGraph& g, const QPlan& p, WaveState& s, 
         --d;
         continue;
       }
-      steps += kTrial ? window_cost(g, p, s, dd, false) : 1u;
+      steps += kMode != kModeCount ? window_cost(g, p, s, dd, false) : 1u;
       const uint32_t i = cur + lane;
       const bool ok = i < s.sl[dd] && passes(g, p, s, dd, g.ladj[s.sb[dd] + i]);
       s.mask[dd] = wave_ballot(ok);
@@ -356,10 +393,10 @@ __device__ uint64_t count_root(const DevGraph& g, const QPlan& p, WaveState& s, 
     s.mask[dd] = m & (m - 1);
     s.emb[dd] = uniform(g.ladj[s.sb[dd] + s.base[dd] + j]);
     if (dd + 1 == p.tail) {
-      total += count_tail<kTrial>(g, p, s, steps);
+      total += count_tail<kMode>(g, p, s, steps);
     } else {
       init_depth(g, p, s, dd + 1);
-      steps += kTrial ? 2u * __builtin_popcount(p.bn[dd + 1]) : 1u;
+      steps += kMode != kModeCount ? 2u * __builtin_popcount(p.bn[dd + 1]) : 1u;
       ++d;
     }
   }
@@ -392,7 +429,7 @@ struct HeavyTask {
 // task past kTrialBudget (its counts are then partial and unused), and
 // adds, per plan, the total to work[3 qid], the largest task's to
 // work[3 qid + 1] and the number of stopped tasks to work[3 qid + 2].
-template <bool kTrial>
+template <int kMode>
 __global__ __launch_bounds__(kBlock) void match_kernel(
     DevGraph g, const QPlan* __restrict__ plans, uint32_t nq, uint32_t n_tasks,
     const uint32_t* __restrict__ roots, const uint32_t* __restrict__ rlen,
@@ -414,7 +451,7 @@ __global__ __launch_bounds__(kBlock) void match_kernel(
       const HeavyTask h = heavy[t];
       const QPlan& p = plans[uniform(h.plan)];
       qid = p.qid;
-      c = count_root<kTrial>(g, p, s, uniform(roots[p.root_off + uniform(h.root)]),
+      c = count_root<kMode>(g, p, s, uniform(roots[p.root_off + uniform(h.root)]),
                              uniform(h.lo), uniform(h.hi), steps);
     } else {
       const uint32_t pi = uniform(find_plan(plans, nq, t - n_heavy));
@@ -424,17 +461,24 @@ __global__ __launch_bounds__(kBlock) void match_kernel(
       const uint32_t r1 = min(r0 + p.chunk, p.n_roots);
       for (uint32_t r = r0; r < r1; ++r) {
         if (p.tail > 1 && uniform(rlen[p.root_off + r]) > split_len) continue;  // a heavy task
-        c += count_root<kTrial>(g, p, s, uniform(roots[p.root_off + r]), 0u, 0xFFFFFFFFu,
+        c += count_root<kMode>(g, p, s, uniform(roots[p.root_off + r]), 0u, 0xFFFFFFFFu,
                                 steps);
       }
     }
     if (lane == 0 && c) atomicAdd(&counts[qid], (unsigned long long)c);
-    if (kTrial && lane == 0 && steps) {
+    if (kMode == kModeLog && lane == 0) work[t] = steps;
+    if (kMode == kModeTrial && lane == 0 && steps) {
       atomicAdd(&work[3 * qid], (unsigned long long)steps);
       atomicMax(&work[3 * qid + 1], (unsigned long long)steps);
       if (steps >= kTrialBudget) atomicAdd(&work[3 * qid + 2], 1ull);
     }
   }
+}
+
+__global__ void zero_f32_kernel(float* a, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    a[i] = 0.f;
 }
 
 __global__ void zero_kernel(unsigned long long* a, int na, unsigned int* b, int nb) {
@@ -741,10 +785,12 @@ struct MatchBatch {
   uint32_t* d_roots = nullptr;
   uint32_t* d_rlen = nullptr;
   unsigned long long* d_counts = nullptr;
-  unsigned long long* d_work = nullptr;  // wave steps per counter (plan trials only)
+  unsigned long long* d_work = nullptr;  // wave steps per counter (plan trials, root logs)
+  int mode = kModeCount;
   unsigned int* d_rootcnt = nullptr;
   unsigned int* d_queue = nullptr;
   HeavyTask* d_heavy = nullptr;  // own allocation
+  unsigned long long* d_log = nullptr;  // own allocation (root logs)
   uint32_t n_heavy = 0, split_len = 0;
   uint32_t nq = 0, n_tasks = 0;
   int n_queries = 0;
@@ -761,6 +807,7 @@ void free_batch(void* p) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   (void)hipFree(b->mem);
   (void)hipFree(b->d_heavy);
+  (void)hipFree(b->d_log);
   delete b;
 }
 
@@ -774,6 +821,12 @@ std::vector<uint32_t> batch_key(const gpm_query* queries, int n_queries, const g
   k.push_back((uint32_t)t.policy);
   k.push_back(t.chunk);
   k.push_back((uint32_t)t.use_graph);
+  if (t.root_model) {
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(t.root_model);
+    k.insert(k.end(), w, w + kMlpWeights);
+  } else {
+    k.push_back(0xFFFFFFFFu);
+  }
   for (int i = 0; i < n_queries; ++i) {
     k.push_back(queries[i].n);
     k.push_back(queries[i].n_edges);
@@ -791,7 +844,10 @@ int launch_batch(gpm_graph* g, MatchBatch* b) {
   }
   hipLaunchKernelGGL(zero_kernel, dim3((b->n_queries + 64) / 64 + 1), dim3(64), 0, g->stream,
                      b->d_counts, b->n_queries, b->d_queue, 4);
-  hipLaunchKernelGGL(b->d_work ? match_kernel<true> : match_kernel<false>, dim3(b->grid),
+  hipLaunchKernelGGL(b->mode == kModeTrial ? match_kernel<kModeTrial>
+                                            : b->mode == kModeLog ? match_kernel<kModeLog>
+                                                                  : match_kernel<kModeCount>,
+                     dim3(b->grid),
                      dim3(kBlock), 0, g->stream, b->dg,
                      b->d_plans, b->nq, b->n_tasks, b->d_roots, b->d_rlen, b->d_heavy,
                      b->n_heavy, b->split_len, b->d_queue, b->d_counts, b->d_work);
@@ -818,7 +874,7 @@ int build_graph(MatchBatch* b) {
                    &b->d_roots, &b->d_rlen,    &b->d_heavy, &b->n_heavy,
                    &b->split_len, &b->d_queue, &b->d_counts, &b->d_work};
   hipKernelNodeParams mp{};
-  mp.func = reinterpret_cast<void*>(&match_kernel<false>);
+  mp.func = reinterpret_cast<void*>(&match_kernel<kModeCount>);
   mp.gridDim = dim3(b->grid);
   mp.blockDim = dim3(kBlock);
   mp.sharedMemBytes = 0;
@@ -869,6 +925,53 @@ bool in_sample(uint32_t v, uint32_t lo, uint32_t stride) {
   return stride <= 1 || root_hash(v ^ root_hash(lo + 0x9e3779b9U)) % stride == 0;
 }
 
+// Predicted weighted DFS steps of every root candidate of the plans on the
+// device (d_plans with root_off set, roots and per-plan counts already
+// filled): features, then the packed MLP at `model` (log1p domain, clamped
+// and mapped back).  cost[root_off + r] for r < rootcnt[plan].
+int predict_root_costs(gpm_graph* g, const DevGraph& dg, const QPlan* d_plans, uint32_t nq,
+                       const uint32_t* d_roots, const unsigned int* d_rootcnt,
+                       uint64_t root_total, uint32_t max_roots, const float* model,
+                       std::vector<float>* cost) {
+  cost->assign(root_total, 0.f);
+  if (!root_total || !nq) return GPM_OK;
+  float *d_w = nullptr, *d_feat = nullptr, *d_pred = nullptr;
+  auto cleanup = [&]() {
+    (void)hipFree(d_w);
+    (void)hipFree(d_feat);
+    (void)hipFree(d_pred);
+  };
+  if (hipMalloc(&d_w, kMlpWeights * 4) != hipSuccess ||
+      hipMalloc(&d_feat, root_total * 16) != hipSuccess ||
+      hipMalloc(&d_pred, root_total * 4) != hipSuccess) {
+    cleanup();
+    return fail(GPM_ENOMEM, "root-cost prediction buffers (%llu roots)",
+                (unsigned long long)root_total);
+  }
+  int st = h2d(g, d_w, model, kMlpWeights * 4);
+  if (st == GPM_OK) {
+    hipLaunchKernelGGL(zero_f32_kernel, dim3((unsigned)std::min<uint64_t>((4 * root_total + 255) / 256,
+                                                                          65535)),
+                       dim3(256), 0, g->stream, d_feat, 4 * root_total);
+    const unsigned gx = (unsigned)std::max<uint64_t>(
+        1, std::min<uint64_t>(((uint64_t)max_roots * kWave + kBlock - 1) / kBlock, 1024));
+    hipLaunchKernelGGL(root_features_kernel, dim3(gx, nq), dim3(kBlock), 0, g->stream, dg,
+                       d_plans, d_roots, d_rootcnt, d_feat);
+    if (hipGetLastError() != hipSuccess) st = fail(GPM_EHIP, "root features launch");
+  }
+  if (st == GPM_OK) st = mlp_forward(d_feat, (uint32_t)root_total, d_w, d_pred, device_cus(g),
+                                     g->stream);
+  if (st == GPM_OK &&
+      (hipMemcpyAsync(cost->data(), d_pred, root_total * 4, hipMemcpyDeviceToHost, g->stream) !=
+           hipSuccess ||
+       hipStreamSynchronize(g->stream) != hipSuccess))
+    st = fail(GPM_EHIP, "root-cost prediction");
+  cleanup();
+  if (st != GPM_OK) return st;
+  for (float& c : *cost) c = std::expm1(std::min(std::max(c, 0.f), 60.f));
+  return GPM_OK;
+}
+
 // Lays out a batch of `plans` (plan.qid = index of its counter; n_counters
 // counters): device buffers, root candidates, chunks (roots per task) and
 // the parts of heavy roots.
@@ -883,7 +986,8 @@ bool in_sample(uint32_t v, uint32_t lo, uint32_t stride) {
 // plan i.  Sampling parts rather than roots keeps
 // hubs represented without running them whole.
 int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gpm_tuning& tun,
-                MatchBatch* b, std::vector<double>* scale) {
+                MatchBatch* b, std::vector<double>* scale, const float* model = nullptr,
+                bool log_roots = false) {
   const bool trial = scale != nullptr;
   const uint32_t nq0 = (uint32_t)plans.size();
   uint64_t root_total = 0;
@@ -915,6 +1019,7 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
   mp += count_bytes;
   if (trial) {
     b->d_work = reinterpret_cast<unsigned long long*>(mp);
+    b->mode = kModeTrial;
     mp += work_bytes;
   }
   b->d_rootcnt = reinterpret_cast<unsigned int*>(mp);
@@ -1016,6 +1121,22 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
   std::vector<double> cost_per_root(nq, 1.0);
   if (tun.autotune && !trial)
     for (uint32_t i = 0; i < nq; ++i) cost_per_root[i] = estimate_root_cost(g, plans[i]);
+  // Learned root-cost model: predicted weighted steps of every root; they
+  // size chunks, order queries and decide which roots are split, and how.
+  std::vector<float> rcost;
+  const bool learned = model && !trial && !log_roots;
+  if (learned) {
+    uint32_t max_roots = 0;
+    for (const QPlan& q : plans) max_roots = std::max(max_roots, q.n_roots);
+    if ((st = predict_root_costs(g, b->dg, b->d_plans, nq, b->d_roots, b->d_rootcnt, root_total,
+                                 max_roots, model, &rcost)) != GPM_OK)
+      return st;
+    for (uint32_t i = 0; i < nq; ++i) {
+      double sum = 0.0;
+      for (uint32_t r = 0; r < plans[i].n_roots; ++r) sum += rcost[plans[i].root_off + r];
+      cost_per_root[i] = std::max(1.0, plans[i].n_roots ? sum / plans[i].n_roots : 1.0);
+    }
+  }
   // Chunks (roots per task) and layout, heaviest queries first when tuned.
   // A tuned chunk gives a task about kTaskWork estimated entries, but no
   // more roots than keep kTasksPerWave tasks per resident wave in the batch:
@@ -1036,10 +1157,21 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
   std::vector<QPlan> laid;
   laid.reserve(nq);
   uint32_t t = 0;
+  // Learned: a task's target is the batch's predicted work over four tasks
+  // per resident wave; light roots are chunked to about that much.
+  double target = 0.0;
+  if (learned) {
+    double total = 0.0;
+    for (float c : rcost) total += c;
+    target = std::max(1.0, total / (4.0 * (double)std::max<uint64_t>(waves, 1)));
+  }
   for (uint32_t i : perm) {
     QPlan q = plans[i];
-    if (trial)
+    if (trial || log_roots)
       q.chunk = 1;
+    else if (learned && !tun.chunk)
+      q.chunk = (uint32_t)std::min(std::min(256.0, chunk_cap),
+                                   std::max(1.0, target / cost_per_root[i]));
     else if (tun.chunk)
       q.chunk = tun.chunk;
     else if (tun.autotune)
@@ -1055,14 +1187,30 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
   // of kSplitLen depth-1 entries each, so one hub's subtree is spread over
   // many waves; parts of the longest slices go first.  Queries whose tail
   // starts at depth 1 are counted per root without a DFS and are not split.
-  b->split_len = kSplitLen;
+  b->split_len = log_roots ? 0xFFFFFFFFu : kSplitLen;
   std::vector<HeavyTask> heavy;
-  std::vector<uint32_t> hlen;
-  for (uint32_t li = 0; li < nq; ++li) {
+  std::vector<double> hlen;  // sort key: slice length, or predicted part cost
+  std::vector<uint32_t> marks;  // learned: UINT32_MAX for a root dealt out as parts
+  if (learned) marks.assign(std::max<uint64_t>(root_total, 1), 0u);
+  for (uint32_t li = 0; li < nq && !log_roots; ++li) {
     const QPlan& q = laid[li];
     if (q.tail <= 1) continue;
     for (uint32_t r = 0; r < q.n_roots; ++r) {
       const uint32_t len = rlen[q.root_off + r];
+      if (learned) {
+        // Parts of about `target` predicted work each, at most one per
+        // depth-1 entry; roots predicted below the target stay whole.
+        const double c = rcost[q.root_off + r];
+        if (len <= 1 || c <= target) continue;
+        const uint32_t np = (uint32_t)std::min<double>(len, std::ceil(c / target));
+        const uint32_t step = (len + np - 1) / np;
+        marks[q.root_off + r] = 0xFFFFFFFFu;
+        for (uint32_t lo = 0; lo < len; lo += step) {
+          heavy.push_back({li, r, lo, std::min(len, lo + step)});
+          hlen.push_back(c * (double)(std::min(len, lo + step) - lo) / (double)len);
+        }
+        continue;
+      }
       if (len <= kSplitLen) continue;
       for (uint32_t lo = 0; lo < len; lo += kSplitLen) {
         if (trial && !sampled(q.qid, troots[q.root_off + r], lo)) continue;
@@ -1070,6 +1218,12 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
         hlen.push_back(len);
       }
     }
+  }
+  if (learned) {
+    // The kernel skips roots whose rlen exceeds split_len in the light
+    // chunks: mark exactly the roots dealt out as parts.
+    b->split_len = 0;
+    if (root_total && (st = h2d(g, b->d_rlen, marks.data(), root_total * 4)) != GPM_OK) return st;
   }
   {
     std::vector<uint32_t> order(heavy.size());
@@ -1090,7 +1244,80 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
   }
   b->n_tasks = b->n_heavy + t;
   if ((st = h2d(g, b->d_plans, laid.data(), (size_t)nq * sizeof(QPlan))) != GPM_OK) return st;
-  if (tun.use_graph && !trial) return build_graph(b);
+  if (log_roots) {
+    // One counter per task (= per root: chunk 1, no parts).
+    if (hipMalloc(&b->d_log, std::max<uint64_t>(b->n_tasks, 1) * 8) != hipSuccess)
+      return fail(GPM_ENOMEM, "root work log");
+    b->d_work = b->d_log;
+    b->mode = kModeLog;
+  }
+  if (tun.use_graph && !trial && !log_roots) return build_graph(b);
+  return GPM_OK;
+}
+
+// Predicted total work of every plan (sum over its root candidates of the
+// model's predicted weighted steps): roots are filtered on the device as for
+// a batch, then featurised and scored by the packed MLP at `model`.
+int plan_costs_by_model(gpm_graph* g, std::vector<QPlan> plans, const float* model,
+                        std::vector<double>* total) {
+  const uint32_t nq = (uint32_t)plans.size();
+  total->assign(nq, 0.0);
+  if (!nq) return GPM_OK;
+  uint64_t root_total = 0;
+  uint32_t max_class = 0;
+  for (QPlan& p : plans) {
+    p.root_off = (uint32_t)root_total;
+    const uint32_t cls = g->h_lvoff[p.label[0] + 1] - g->h_lvoff[p.label[0]];
+    root_total += cls;
+    max_class = std::max(max_class, cls);
+  }
+  if (root_total >= (1ull << 32)) return fail(GPM_EINVAL, "root candidate total too large");
+  QPlan* d_plans = nullptr;
+  uint32_t *d_roots = nullptr, *d_rlen = nullptr, *d_lvoff = nullptr;
+  unsigned int* d_cnt = nullptr;
+  auto cleanup = [&]() {
+    (void)hipFree(d_plans);
+    (void)hipFree(d_roots);
+    (void)hipFree(d_rlen);
+    (void)hipFree(d_lvoff);
+    (void)hipFree(d_cnt);
+  };
+  if (hipMalloc(&d_plans, nq * sizeof(QPlan)) != hipSuccess ||
+      hipMalloc(&d_roots, std::max<uint64_t>(root_total, 1) * 4) != hipSuccess ||
+      hipMalloc(&d_rlen, std::max<uint64_t>(root_total, 1) * 4) != hipSuccess ||
+      hipMalloc(&d_lvoff, g->h_lvoff.size() * 4) != hipSuccess ||
+      hipMalloc(&d_cnt, nq * 4) != hipSuccess) {
+    cleanup();
+    return fail(GPM_ENOMEM, "plan scoring buffers");
+  }
+  const DevGraph dg{g->off, g->ladj, g->loff, g->n_labels, g->sig};
+  int st = h2d(g, d_plans, plans.data(), nq * sizeof(QPlan));
+  if (st == GPM_OK) st = h2d(g, d_lvoff, g->h_lvoff.data(), g->h_lvoff.size() * 4);
+  std::vector<unsigned int> cnt(nq, 0);
+  std::vector<float> cost;
+  if (st == GPM_OK) {
+    hipLaunchKernelGGL(zero_kernel, dim3((nq + 64) / 64 + 1), dim3(64), 0, g->stream, nullptr, 0,
+                       d_cnt, (int)nq);
+    dim3 rgrid((unsigned)std::max<uint32_t>(1, std::min<uint32_t>((max_class + kBlock - 1) / kBlock,
+                                                                   256)),
+               nq);
+    hipLaunchKernelGGL(roots_kernel, rgrid, dim3(kBlock), 0, g->stream, dg, g->lverts, d_lvoff,
+                       d_plans, d_roots, d_rlen, d_cnt);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(cnt.data(), d_cnt, nq * 4, hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+        hipStreamSynchronize(g->stream) != hipSuccess)
+      st = fail(GPM_EHIP, "plan scoring roots");
+  }
+  if (st == GPM_OK) {
+    uint32_t max_roots = 0;
+    for (unsigned int c : cnt) max_roots = std::max(max_roots, (uint32_t)c);
+    st = predict_root_costs(g, dg, d_plans, nq, d_roots, d_cnt, root_total, max_roots, model,
+                            &cost);
+  }
+  cleanup();
+  if (st != GPM_OK) return st;
+  for (uint32_t i = 0; i < nq; ++i)
+    for (uint32_t r = 0; r < cnt[i]; ++r) (*total)[i] += cost[plans[i].root_off + r];
   return GPM_OK;
 }
 
@@ -1114,7 +1341,7 @@ int plan_queries(gpm_graph* g, const gpm_query* queries, int n_queries, const gp
     QPlan p;
     if (!make_plan(g, hq, tun.policy, &p)) continue;  // a label the graph lacks: count 0
     p.qid = (uint32_t)qi;
-    if (tun.policy == GPM_SCHED_LEARNED && tun.autotune) {
+    if (tun.policy == GPM_SCHED_LEARNED && (tun.autotune || tun.root_model)) {
       std::vector<QPlan> cand = plan_candidates(g, hq);
       if (cand.size() > 1) {
         tried.push_back({out->size(), trial_plans.size(), cand.size()});
@@ -1127,6 +1354,21 @@ int plan_queries(gpm_graph* g, const gpm_query* queries, int n_queries, const gp
     out->push_back(p);
   }
   if (trial_plans.empty()) return GPM_OK;
+  if (tun.root_model) {
+    // Learned QueryingOrder: the candidate with the least predicted work; a
+    // later candidate must be cheaper by more than kTrialMargin to replace an
+    // earlier (better estimated) one.
+    std::vector<double> total;
+    int st = plan_costs_by_model(g, trial_plans, tun.root_model, &total);
+    if (st != GPM_OK) return st;
+    for (const Tried& tr : tried) {
+      size_t best = tr.first;
+      for (size_t i = tr.first + 1; i < tr.first + tr.count; ++i)
+        if (total[i] < kTrialMargin * total[best]) best = i;
+      (*out)[tr.at] = trial_plans[best];
+    }
+    return GPM_OK;
+  }
   MatchBatch* t = new MatchBatch();
   std::vector<double> scale;
   int st = build_batch(g, trial_plans, 0, tun, t, &scale);
@@ -1181,7 +1423,8 @@ int prepare_batch(gpm_graph* g, const gpm_query* queries, int n_queries, const g
   std::vector<QPlan> plans;
   int st = plan_queries(g, queries, n_queries, tun, &plans);
   if (st != GPM_OK) return st;
-  return build_batch(g, std::move(plans), n_queries, tun, b, nullptr);
+  return build_batch(g, std::move(plans), n_queries, tun, b, nullptr,
+                     tun.policy == GPM_SCHED_LEARNED ? tun.root_model : nullptr);
 }
 
 }  // namespace
@@ -1192,6 +1435,118 @@ void free_match_cache(gpm_graph* g) {
   g->match_cache = nullptr;
 }
 }  // namespace gpm
+
+namespace {
+
+// Validation shared by the entry points: query shape and connectivity.
+int check_query(const gpm_query& gq, int qi) {
+  GPM_CHECK_ARG(gq.n >= 1 && gq.n <= (uint32_t)kMaxQ, "query %d: 1 <= n <= 16 (got %u)", qi, gq.n);
+  GPM_CHECK_ARG(gq.labels && (gq.n_edges == 0 || gq.edges), "query %d: null arrays", qi);
+  HostQuery hq;
+  hq.n = gq.n;
+  hq.adj.assign(gq.n, {});
+  for (uint32_t e = 0; e < gq.n_edges; ++e) {
+    const uint32_t a = gq.edges[2 * e], c = gq.edges[2 * e + 1];
+    GPM_CHECK_ARG(a < gq.n && c < gq.n && a != c, "query %d: bad edge %u (%u, %u)", qi, e, a, c);
+    hq.adj[a].push_back(c);
+    hq.adj[c].push_back(a);
+  }
+  GPM_CHECK_ARG(connected(hq), "query %d is not connected", qi);
+  return GPM_OK;
+}
+
+// One query's plan (policy's order, no trial) laid out one root per task;
+// with `log`, the batch records every task's weighted steps.  *ok = false
+// when a query label is absent from the graph (no roots).
+int single_query_batch(gpm_graph* g, const gpm_query* query, int policy, bool log, MatchBatch* b,
+                       bool* ok) {
+  GPM_CHECK_ARG(g && query, "null pointer");
+  GPM_CHECK_ARG(g->flags & GPM_GRAPH_SYMMETRIC, "matching needs a SYMMETRIC graph");
+  GPM_CHECK_ARG(g->labels && g->ladj, "matching needs a labeled graph");
+  GPM_CHECK_ARG(policy == GPM_SCHED_DEGREE || policy == GPM_SCHED_LEARNED, "unknown policy %d",
+                policy);
+  int st;
+  if ((st = check_query(*query, 0)) != GPM_OK) return st;
+  QPlan p;
+  *ok = make_plan(g, host_query(*query), policy, &p);
+  if (!*ok) return GPM_OK;
+  p.qid = 0;
+  gpm_tuning tun{};
+  tun.chunk = 1;
+  tun.policy = policy;
+  return build_batch(g, {p}, 1, tun, b, nullptr, nullptr, log);
+}
+
+}  // namespace
+
+extern "C" int gpm_match_root_work(gpm_graph* g, const gpm_query* query, int policy,
+                                   uint64_t* work) {
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    GPM_CHECK_ARG(work, "null pointer");
+    MatchBatch b;
+    bool ok = false;
+    int st = single_query_batch(g, query, policy, true, &b, &ok);
+    if (st == GPM_OK) std::fill(work, work + g->n, 0ull);
+    if (st == GPM_OK && ok && b.n_tasks) {
+      st = launch_batch(g, &b);
+      std::vector<unsigned long long> w(b.n_tasks);
+      std::vector<uint32_t> roots(b.n_tasks);
+      if (st == GPM_OK &&
+          (hipMemcpyAsync(w.data(), b.d_work, b.n_tasks * 8, hipMemcpyDeviceToHost, g->stream) !=
+               hipSuccess ||
+           hipMemcpyAsync(roots.data(), b.d_roots, b.n_tasks * 4, hipMemcpyDeviceToHost,
+                          g->stream) != hipSuccess ||
+           hipStreamSynchronize(g->stream) != hipSuccess))
+        st = fail(GPM_EHIP, "root work log");
+      if (st == GPM_OK)
+        for (uint32_t t = 0; t < b.n_tasks; ++t) work[roots[t]] = w[t];
+    }
+    MatchBatch* owned = new MatchBatch(std::move(b));
+    free_batch(owned);
+    return st;
+  });
+}
+
+extern "C" int gpm_match_root_features(gpm_graph* g, const gpm_query* query, int policy,
+                                       float* feat) {
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    GPM_CHECK_ARG(feat, "null pointer");
+    MatchBatch b;
+    bool ok = false;
+    int st = single_query_batch(g, query, policy, true, &b, &ok);  // one root per task
+    if (st == GPM_OK) std::fill(feat, feat + 4 * (size_t)g->n, 0.f);
+    const uint32_t nr = ok ? b.n_tasks : 0;
+    if (st == GPM_OK && nr) {
+      float* d_feat = nullptr;
+      std::vector<float> f(4 * (size_t)nr);
+      std::vector<uint32_t> roots(nr);
+      if (hipMalloc(&d_feat, (size_t)nr * 16) != hipSuccess) {
+        st = fail(GPM_ENOMEM, "root features");
+      } else {
+        const unsigned gx = (unsigned)std::max<uint64_t>(
+            1, std::min<uint64_t>(((uint64_t)nr * kWave + kBlock - 1) / kBlock, 1024));
+        hipLaunchKernelGGL(root_features_kernel, dim3(gx, 1), dim3(kBlock), 0, g->stream, b.dg,
+                           b.d_plans, b.d_roots, b.d_rootcnt, d_feat);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(f.data(), d_feat, f.size() * 4, hipMemcpyDeviceToHost, g->stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(roots.data(), b.d_roots, nr * 4, hipMemcpyDeviceToHost, g->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(g->stream) != hipSuccess)
+          st = fail(GPM_EHIP, "root features");
+        (void)hipFree(d_feat);
+      }
+      if (st == GPM_OK)
+        for (uint32_t r = 0; r < nr; ++r)
+          for (int k = 0; k < 4; ++k) feat[4 * (size_t)roots[r] + k] = f[4 * (size_t)r + k];
+    }
+    MatchBatch* owned = new MatchBatch(std::move(b));
+    free_batch(owned);
+    return st;
+  });
+}
 
 extern "C" int gpm_match_batch(gpm_graph* g, const gpm_query* queries, int n_queries,
                                const gpm_tuning* tuning, uint64_t* per_query) {

@@ -166,7 +166,36 @@ __global__ __launch_bounds__(kBlock) void features_kernel(
   }
 }
 
+Mlp unpack(const float* d_w) {
+  Mlp m;
+  m.mean = d_w;
+  m.std = d_w + 4;
+  m.w1 = d_w + 8;
+  m.b1 = m.w1 + 64 * 4;
+  m.w2 = m.b1 + 64;
+  m.b2 = m.w2 + 64 * 64;
+  m.w3 = m.b2 + 64;
+  m.b3 = m.w3 + 64;
+  return m;
+}
+
 }  // namespace
+
+namespace gpm {
+
+int mlp_forward(const float* d_feat, uint32_t n, const float* d_packed, float* d_out, int cus,
+                hipStream_t stream) {
+  if (!n) return GPM_OK;
+  const uint64_t groups = (n + 15) / 16;
+  const unsigned blocks = (unsigned)std::max<uint64_t>(
+      1, std::min<uint64_t>((groups + kWavesPerBlock - 1) / kWavesPerBlock, (uint64_t)cus * 8));
+  hipLaunchKernelGGL(mlp_kernel, dim3(blocks), dim3(kBlock), 0, stream, d_feat, n,
+                     unpack(d_packed), d_out);
+  GPM_HIP(hipGetLastError());
+  return GPM_OK;
+}
+
+}  // namespace gpm
 
 extern "C" {
 
@@ -206,7 +235,7 @@ int gpm_mlp_predict(const float* feat, uint32_t n, const float* norm_mean, const
                     "normalisation std must be finite and > 0");
     for (uint32_t i = 0; i < 4 * n; ++i)
       GPM_CHECK_ARG(feat[i] > -1.f && std::isfinite(feat[i]), "features must be finite and > -1");
-    const size_t wcount = 4 + 4 + 64 * 4 + 64 + 64 * 64 + 64 + 64 + 1;
+    const size_t wcount = kMlpWeights;
     std::vector<float> packed;
     packed.reserve(wcount);
     auto put = [&](const float* p, size_t c) { packed.insert(packed.end(), p, p + c); };
@@ -230,15 +259,7 @@ int gpm_mlp_predict(const float* feat, uint32_t n, const float* norm_mean, const
     if (e == hipSuccess && n)
       e = hipMemcpyAsync(d_feat, feat, (size_t)n * 16, hipMemcpyHostToDevice, stream);
     if (e == hipSuccess && n) {
-      Mlp m;
-      m.mean = d_w;
-      m.std = d_w + 4;
-      m.w1 = d_w + 8;
-      m.b1 = m.w1 + 64 * 4;
-      m.w2 = m.b1 + 64;
-      m.b2 = m.w2 + 64 * 64;
-      m.w3 = m.b2 + 64;
-      m.b3 = m.w3 + 64;
+      const Mlp m = unpack(d_w);
       hipDeviceProp_t prop;
       int cus = 256;
       if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;

@@ -53,7 +53,7 @@ class Query(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("autotune", C.c_int), ("policy", C.c_int),
                 ("chunk", C.c_uint32), ("use_graph", C.c_int),
-                ("reserved", C.c_uint32 * 4)]
+                ("root_model", _f32p), ("reserved", C.c_uint32 * 2)]
 
 
 _lib = None
@@ -275,6 +275,37 @@ class DeviceGraph:
         _check(lib().gpm_graph_info(self._h, C.byref(n), C.byref(m), C.byref(d)))
         return n.value, m.value, d.value
 
+    @classmethod
+    def build(cls, n: int, src, dst, flags: int = CSR_CLEAN, oriented: bool = False,
+              labels=None, n_labels: int = 0) -> "DeviceGraph":
+        """Device graph built on the GPU from an edge list (gpm_graph_build)."""
+        src, dst = _u32(src), _u32(dst)
+        assert len(src) == len(dst)
+        lab = _u32(labels) if labels is not None else None
+        if lab is not None:
+            n_labels = n_labels or (int(lab.max()) + 1 if len(lab) else 1)
+        h = C.c_void_p()
+        _check(lib().gpm_graph_build(C.c_uint32(n), _ptr(src), _ptr(dst), C.c_uint64(len(src)),
+                                     C.c_uint32(flags), _ptr(lab) if lab is not None else None,
+                                     C.c_uint32(n_labels),
+                                     C.c_uint32(GRAPH_ORIENTED if oriented else GRAPH_SYMMETRIC),
+                                     C.byref(h)))
+        return cls(None, oriented, _handle=h)
+
+    def relabel_by_degree(self):
+        """(new DeviceGraph, perm) with perm[old] = new id (gpm_graph_relabel_by_degree)."""
+        h, pp = C.c_void_p(), _u32p()
+        _check(lib().gpm_graph_relabel_by_degree(self._h, C.byref(h), C.byref(pp)))
+        n = self.info()[0]
+        return DeviceGraph(None, self.oriented, _handle=h), _take(pp, n)
+
+    def to_host(self) -> HostCSR:
+        n, m, _ = self.info()
+        off = np.zeros(n + 1, dtype=np.uint32)
+        adj = np.zeros(max(m, 1), dtype=np.uint32)
+        _check(lib().gpm_graph_copy_csr(self._h, _ptr(off), _ptr(adj)))
+        return HostCSR(off, adj[:m])
+
     def orient(self) -> "DeviceGraph":
         h = C.c_void_p()
         _check(lib().gpm_graph_orient(self._h, C.byref(h)))
@@ -351,10 +382,8 @@ class DeviceGraph:
                                  C.c_int(formulation), out, C.byref(scalar)))
         return [int(x) for x in out], scalar.value
 
-    def match_batch(self, queries: Sequence, autotune: int = 1,
-                    policy: int = SCHED_DEGREE, chunk: int = 0,
-                    use_graph: int = 0) -> np.ndarray:
-        """queries: sequence of (n, edges [(u, v), ...], labels [..])."""
+    @staticmethod
+    def _queries(queries):
         arr = (Query * len(queries))()
         keep = []
         for i, (qn, edges, labels) in enumerate(queries):
@@ -363,12 +392,40 @@ class DeviceGraph:
             keep += [e, l]
             arr[i].n, arr[i].n_edges = qn, len(e) // 2
             arr[i].edges, arr[i].labels = _ptr(e), _ptr(l)
+        return arr, keep
+
+    def match_batch(self, queries: Sequence, autotune: int = 1,
+                    policy: int = SCHED_DEGREE, chunk: int = 0,
+                    use_graph: int = 0, model: Optional[np.ndarray] = None) -> np.ndarray:
+        """queries: sequence of (n, edges [(u, v), ...], labels [..]); model: packed
+        root-cost MLP (4553 float32, see MatchModel) for the LEARNED policy."""
+        arr, keep = self._queries(queries)
         t = Tuning()
         t.autotune, t.policy, t.chunk, t.use_graph = autotune, policy, chunk, use_graph
+        if model is not None:
+            self._model = np.ascontiguousarray(model, dtype=np.float32)
+            assert self._model.shape == (4553,)
+            t.root_model = _ptr(self._model, C.c_float)
         out = np.zeros(len(queries), dtype=np.uint64)
         _check(lib().gpm_match_batch(self._h, arr, C.c_int(len(queries)),
                                      C.byref(t), _ptr(out, C.c_uint64)))
         return out
+
+    def match_root_work(self, query, policy: int = SCHED_DEGREE) -> np.ndarray:
+        """Per-vertex weighted DFS steps of one query's plan (gpm_match_root_work)."""
+        arr, keep = self._queries([query])
+        n = self.info()[0]
+        w = np.zeros(n, dtype=np.uint64)
+        _check(lib().gpm_match_root_work(self._h, arr, C.c_int(policy), _ptr(w, C.c_uint64)))
+        return w
+
+    def match_root_features(self, query, policy: int = SCHED_DEGREE) -> np.ndarray:
+        """n x 4 root features of one query's plan (gpm_match_root_features)."""
+        arr, keep = self._queries([query])
+        n = self.info()[0]
+        f = np.zeros(4 * n, dtype=np.float32)
+        _check(lib().gpm_match_root_features(self._h, arr, C.c_int(policy), _ptr(f, C.c_float)))
+        return f.reshape(n, 4)
 
 
 def tc_edge_noalloc(d_offsets: int, d_adj: int, n: int, m: int, d_count: int,

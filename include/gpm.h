@@ -180,6 +180,28 @@ int gpm_graph_info(const gpm_graph* g, uint32_t* n, uint64_t* m,
  */
 int gpm_graph_orient(gpm_graph* sym, gpm_graph** dag);
 
+/*
+ * Builds a device graph from an edge list on the GPU, replacing the host
+ * passes of Graph::Init (src/graph/graph.h:254-340): csr_flags as for
+ * gpm_csr_from_edges (SYMMETRIZE, DEDUP, DROP_SELF, SORT; DEDUP implies
+ * SORT; without SORT every list keeps edge-list order) -- the same CSR
+ * gpm_csr_from_edges returns, built with device radix sorts.  labels /
+ * n_labels / graph_flags as for gpm_graph_create.
+ */
+int gpm_graph_build(uint32_t n, const uint32_t* src, const uint32_t* dst, uint64_t n_edges,
+                    uint32_t csr_flags, const uint32_t* labels, uint32_t n_labels,
+                    uint32_t graph_flags, gpm_graph** out);
+/*
+ * Degree relabel on the GPU, replacing Graph::SortCSRGraph
+ * (src/graph/graph.h:206-252) and matching gpm_csr_sort_by_degree: vertices
+ * stable-sorted by list length, descending; *out is a new handle over the
+ * renamed graph (lists sorted, labels carried over) and *perm (n entries,
+ * release with gpm_free) holds new_id[old].
+ */
+int gpm_graph_relabel_by_degree(gpm_graph* g, gpm_graph** out, uint32_t** perm);
+/* Copies the device CSR to caller buffers of n + 1 and m entries. */
+int gpm_graph_copy_csr(const gpm_graph* g, uint32_t* offsets, uint32_t* adj);
+
 /* Event-timed duration (ms) of the kernels of the last mining call on g. */
 int gpm_last_kernel_ms(const gpm_graph* g, double* ms);
 /*
@@ -363,13 +385,22 @@ typedef struct gpm_query {
 
 typedef struct gpm_tuning {
   int autotune;             /* 1: project autotuner picks per-query chunks;
-                               with policy LEARNED also the plan, by one
-                               trial launch over sampled tasks              */
+                               with policy LEARNED and no root_model also the
+                               plan, by one trial launch over sampled tasks */
   int policy;               /* gpm_sched_policy for the matching order      */
   uint32_t chunk;           /* candidate roots per task, 0 = automatic       */
   int use_graph;            /* 1: replay the batch as one hipGraph; needs
                                autotune = 0 (else GPM_EUNSUPPORTED)          */
-  uint32_t reserved[4];
+  const float* root_model;  /* LEARNED: packed root-cost MLP (mean[4] std[4]
+                               W1[64x4] b1[64] W2[64x64] b2[64] W3[64] b3[1],
+                               4553 floats) predicting log1p of a root's
+                               weighted DFS steps from (degree, depth-1
+                               slice length, 2-hop degree sum, query size);
+                               it picks the matching order among the
+                               candidate orders and sizes chunks and
+                               heavy-root parts.  NULL: label statistics and
+                               the plan trial instead                        */
+  uint32_t reserved[2];
 } gpm_tuning;
 
 /*
@@ -382,6 +413,18 @@ typedef struct gpm_tuning {
  */
 int gpm_match_batch(gpm_graph* g, const gpm_query* queries, int n_queries,
                     const gpm_tuning* tuning, uint64_t* per_query);
+
+/*
+ * Training data of the learned matching model (no counterpart in the
+ * reference, whose QueryingOrder.h is empty).  For the plan `policy` picks
+ * for one query (no plan trial): work[v] = weighted DFS steps (windows and
+ * descents weighed by their dependent loads) the exact miner spends below
+ * root candidate v, 0 for other vertices (n entries); feat[4v .. 4v+3] = the
+ * model's features of root candidate v (degree, depth-1 label-slice length,
+ * degrees summed over that slice, query size), zeros elsewhere (4n).
+ */
+int gpm_match_root_work(gpm_graph* g, const gpm_query* query, int policy, uint64_t* work);
+int gpm_match_root_features(gpm_graph* g, const gpm_query* query, int policy, float* feat);
 
 /* ------------------------------------------------------- cost model (MLP) */
 

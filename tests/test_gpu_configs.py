@@ -105,12 +105,12 @@ def test_config5_motif4_rmat22_identities(gpm, rmat22):
         star, path, tailed, cycle, diamond, k4 = sym.pattern("Motif4", gpm.FORM_PATTERN)[0]
         tri = dag.clique(3)
         k4_dag = dag.clique(4)
-    stars = sum(int(x) * (int(x) - 1) * (int(x) - 2) // 6 for x in g.degrees().tolist())
-    src = np.repeat(np.arange(g.n), g.degrees())
-    dd = g.degrees().astype(np.int64)
-    a, b = dd[src] - 1, dd[g.adj.astype(np.int64)] - 1
-    keep = src < g.adj  # each undirected edge once
-    paths = int(np.sum(a[keep].astype(object) * b[keep].astype(object))) - 3 * tri
+    dd = g.degrees().astype(np.int64)  # products and sums below fit in int64
+    stars = int(np.sum(dd * (dd - 1) * (dd - 2) // 6))
+    src = np.repeat(np.arange(g.n, dtype=np.int64), dd)
+    dst = g.adj.astype(np.int64)
+    keep = src < dst  # each undirected edge once
+    paths = int(np.sum((dd[src[keep]] - 1) * (dd[dst[keep]] - 1))) - 3 * tri
     assert stars == star + tailed + 2 * diamond + 4 * k4
     assert paths == path + 2 * tailed + 4 * cycle + 6 * diamond + 12 * k4
     assert k4 == k4_dag

@@ -144,3 +144,59 @@ def test_heavy_roots_split(gpm, orc):
             assert np.array_equal(dg.match_batch(qs, autotune=0, chunk=2, use_graph=1), want)
     # Closed form on the wheel: 6 (n - 1) ordered triangles.
     assert want[0] != 0
+
+
+def _random_model(seed):
+    """A packed root-cost MLP with random weights: the learned policy must
+    count exactly whatever the model predicts (it only picks orders and
+    sizes tasks)."""
+    rng = np.random.default_rng(seed)
+    parts = [rng.normal(0, 1, 4), rng.uniform(0.5, 2.0, 4), rng.normal(0, 0.5, 256),
+             rng.normal(0, 0.1, 64), rng.normal(0, 0.125, 4096), rng.normal(0, 0.1, 64),
+             rng.normal(0, 0.125, 64), rng.normal(2.0, 1.0, 1)]
+    return np.concatenate(parts).astype(np.float32)
+
+
+def _models():
+    import os
+    from cost_model import CostModel, default_path
+    out = [_random_model(1), _random_model(2)]
+    if os.path.exists(default_path(0)):
+        out.append(CostModel.load_default(0).packed())
+    return out
+
+
+def test_learned_model_counts_exact(gpm, hprd, orc):
+    """LEARNED with a root-cost model (order picked by predicted work, chunks
+    and heavy-root parts sized by it) equals the reference's known answers on
+    HPRD and the oracle on a labeled R-MAT graph with hubs."""
+    dg, qs, expected = hprd
+    for model in _models():
+        assert np.array_equal(dg.match_batch(qs, policy=1, model=model), expected)
+    rng = np.random.default_rng(11)
+    g = gpm.rmat_graph(11, 16, 321)
+    labels = rng.integers(0, 3, g.n).astype(np.uint32)
+    qs = _random_queries(rng, 3)
+    want = orc.ceci_batch(g.offsets, g.adj, labels, qs)
+    with gpm.DeviceGraph(g, oriented=False, labels=labels, n_labels=3) as d2:
+        for model in _models():
+            assert np.array_equal(d2.match_batch(qs, policy=1, model=model), want)
+            assert np.array_equal(d2.match_batch(qs, policy=1, autotune=0, model=model), want)
+
+
+def test_root_work_and_features(gpm):
+    g = gpm.rmat_graph(10, 8, 5)
+    labels = np.zeros(g.n, dtype=np.uint32)
+    tri = (3, np.array([[0, 1], [1, 2], [0, 2]], dtype=np.uint32), np.zeros(3, np.uint32))
+    with gpm.DeviceGraph(g, oriented=False, labels=labels, n_labels=1) as dg:
+        f = dg.match_root_features(tri)
+        w = dg.match_root_work(tri)
+    deg = g.degrees()
+    cand = deg >= 2  # triangle roots need degree >= 2 (one label: NLF = degree)
+    assert np.array_equal(f[:, 3] > 0, cand)
+    assert np.array_equal(f[cand, 0], deg[cand].astype(np.float32))
+    assert np.array_equal(f[cand, 1], deg[cand].astype(np.float32))  # the one label's slice
+    src = np.repeat(np.arange(g.n), deg)
+    hop = np.bincount(src, weights=deg[g.adj.astype(np.int64)], minlength=g.n)
+    assert np.array_equal(f[cand, 2], hop[cand].astype(np.float32))
+    assert np.all(w[~cand] == 0) and np.all(w[cand] > 0)
