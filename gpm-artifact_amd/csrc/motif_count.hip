@@ -8,8 +8,9 @@
 //   N_cycle   = 4-cycles, counted per max-rank corner (below)
 //   N_diamond = sum_{edges e} C(c_e, 2)        (c_e = triangles on e)
 //   N_K4      = 4-cliques of the degree-oriented DAG
-// The edge sums take every undirected edge once (the DAG's edges) and
-// compute c_e on the fly.
+// The edge sums take every undirected edge once (the DAG's edges); c_e
+// comes from listing every triangle once on the DAG (its first edge and apex)
+// and adding one to each of its three edges.
 // and the induced counts by inclusion-exclusion over how many copies of
 // each shape a larger shape holds:
 //   I_K4 = N_K4
@@ -49,47 +50,64 @@ unsigned grid_of(const gpm_graph* g, uint64_t threads) {
   return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)device_cus(g) * 32));
 }
 
-// Edge sums over the undirected edges {u, v} (each once: the edges of the
-// degree-oriented DAG), with c = |N(u) & N(v)| (triangles on the edge),
-// counted by G lanes per edge -- each element of the shorter list is
-// searched in the longer one -- and folded in at once:
+// Triangles per DAG edge (c_e = |N(u) & N(v)| for the undirected edge of e):
+// every triangle is found once, from its first DAG edge (u, v) and its apex
+// w in N+(u) & N+(v) -- G lanes per edge, each taking entries of N+(u) and
+// searching N+(v) -- and adds one to each of its three edges: (u, w) and
+// (v, w) by their positions in the DAG lists, (u, v) once per edge with the
+// group's total.  tri must be zero on entry; m_dag entries.
+template <int G>
+__global__ __launch_bounds__(kBlock) void edge_triangles_kernel(
+    const uint32_t* __restrict__ doff, const uint32_t* __restrict__ dadj,
+    const uint32_t* __restrict__ dsrc, uint64_t m, uint32_t* __restrict__ tri) {
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t sub = threadIdx.x & (G - 1);
+  const uint64_t ngrp = (uint64_t)gridDim.x * blockDim.x / G;
+  for (uint64_t e = tid / G; e < m; e += ngrp) {
+    const uint32_t u = dsrc[e], v = dadj[e];
+    const uint32_t ub = doff[u], nu = doff[u + 1] - ub;
+    const uint32_t vb = doff[v], nv = doff[v + 1] - vb;
+    uint32_t cnt = 0;
+    for (uint32_t i = sub; i < nu; i += G) {
+      const uint32_t w = dadj[ub + i];
+      // position of w in N+(v) (lower bound), if present
+      uint32_t lo = 0, len = nv;
+      while (len > 1) {
+        const uint32_t half = len >> 1;
+        lo = dadj[vb + lo + half] <= w ? lo + half : lo;
+        len -= half;
+      }
+      if (nv && dadj[vb + lo] == w) {
+        atomicAdd(&tri[ub + i], 1u);
+        atomicAdd(&tri[vb + lo], 1u);
+        ++cnt;
+      }
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, kWave);
+    if (sub == 0 && cnt) atomicAdd(&tri[e], cnt);
+  }
+}
+
+// Edge sums over the undirected edges {u, v} (each once: the DAG's edges),
+// from the per-edge triangle counts c and the symmetric degrees:
 //   out[0] += 2c                      (sum: 6T)
 //   out[2] += c (d_u + d_v - 4)       (sum: 2 N_paw)
 //   out[3] += c (c - 1)               (sum: 2 N_diamond)
 //   out[4] += 2 (d_u - 1)(d_v - 1)    (sum: 2 x the path edge sum)
-template <int G>
-__global__ __launch_bounds__(kBlock) void edge_sums_kernel(
-    const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
-    const uint32_t* __restrict__ esrc, const uint32_t* __restrict__ edst, uint64_t m,
+__global__ __launch_bounds__(kBlock) void edge_stats_kernel(
+    const uint32_t* __restrict__ off, const uint32_t* __restrict__ dsrc,
+    const uint32_t* __restrict__ dadj, const uint32_t* __restrict__ tri, uint64_t m,
     unsigned long long* __restrict__ out) {
-  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t sub = threadIdx.x & (G - 1);
-  const uint64_t ngrp = (uint64_t)gridDim.x * blockDim.x / G;
   uint64_t s6t = 0, spaw = 0, sdia = 0, spath = 0;
-  for (uint64_t e = tid / G; e < m; e += ngrp) {
-    const uint32_t u = esrc[e], v = edst[e];
-    uint32_t ab = off[u], na = off[u + 1] - ab;
-    uint32_t bb = off[v], nb = off[v + 1] - bb;
-    if (na > nb) {
-      uint32_t t = ab;
-      ab = bb;
-      bb = t;
-      t = na;
-      na = nb;
-      nb = t;
-    }
-    uint32_t cnt = 0;
-    for (uint32_t i = sub; i < na; i += G) cnt += sorted_contains(adj + bb, nb, adj[ab + i]);
-    // Sum over the G lanes of this group (aligned, power of two).
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, kWave);
-    if (sub == 0) {
-      const uint64_t c = cnt, da = na, db = nb;
-      s6t += 2 * c;
-      spaw += c ? c * (da + db - 4) : 0;
-      sdia += c * (c - (c ? 1 : 0));
-      spath += 2 * (da - 1) * (db - 1);
-    }
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t u = dsrc[e], v = dadj[e];
+    const uint64_t c = tri[e], da = off[u + 1] - off[u], db = off[v + 1] - off[v];
+    s6t += 2 * c;
+    spaw += c ? c * (da + db - 4) : 0;
+    sdia += c * (c - (c ? 1 : 0));
+    spath += 2 * (da - 1) * (db - 1);
   }
   s6t = wave_sum_u64(s6t);
   spaw = wave_sum_u64(spaw);
@@ -319,18 +337,22 @@ int motif4_by_identities(gpm_graph* g, uint64_t out[6]) {
   unsigned blocks_c = (unsigned)std::min<uint64_t>((uint64_t)cus * 2, std::max(g->n, 1u));
   while (blocks_c > 16 && (uint64_t)blocks_c * stride * 4 > (8ull << 30)) blocks_c /= 2;
   const uint64_t cyc_bytes = (uint64_t)blocks_c * stride * 4;
-  if ((st = ensure_scratch(g, cyc_bytes + 256)) != GPM_OK) return st;
+  // The per-edge triangle counts (one uint32 per DAG edge) use the same
+  // scratch before the 4-cycle counters do.
+  const uint64_t tri_bytes = std::max<uint64_t>(g->dag->m, 1) * 4;
+  if ((st = ensure_scratch(g, std::max(cyc_bytes, tri_bytes) + 256)) != GPM_OK) return st;
+  uint32_t* tri = static_cast<uint32_t*>(g->scratch);
   uint32_t* ccnt = static_cast<uint32_t*>(g->scratch);
   if ((st = timing_begin(g)) != GPM_OK) return st;
   if ((st = zero_counters(g, 16)) != GPM_OK) return st;
   unsigned int* queue = reinterpret_cast<unsigned int*>(g->counters + 8);
-  const uint64_t zero_n = cyc_bytes / 4;
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_of(g, zero_n)), dim3(256), 0, g->stream, ccnt,
-                     zero_n);
   const uint64_t me = g->dag->m;
   if (me) {
-    hipLaunchKernelGGL(edge_sums_kernel<8>, dim3(grid_of(g, me * 8)), dim3(kBlock), 0, g->stream,
-                       g->off, g->adj, g->dag->src, g->dag->adj, me, g->counters);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_of(g, me)), dim3(256), 0, g->stream, tri, me);
+    hipLaunchKernelGGL(edge_triangles_kernel<8>, dim3(grid_of(g, me * 8)), dim3(kBlock), 0,
+                       g->stream, g->dag->off, g->dag->adj, g->dag->src, me, tri);
+    hipLaunchKernelGGL(edge_stats_kernel, dim3(grid_of(g, me)), dim3(kBlock), 0, g->stream,
+                       g->off, g->dag->src, g->dag->adj, tri, me, g->counters);
     GPM_HIP(hipGetLastError());
   }
   timing_mark(g, "motif_edge_sums");
@@ -338,6 +360,10 @@ int motif4_by_identities(gpm_graph* g, uint64_t out[6]) {
                      g->n, g->counters);
   GPM_HIP(hipGetLastError());
   timing_mark(g, "motif_star_sum");
+  // The 4-cycle counters start at zero (the triangle counts lived here).
+  const uint64_t zero_n = cyc_bytes / 4;
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_of(g, zero_n)), dim3(256), 0, g->stream, ccnt,
+                     zero_n);
   hipLaunchKernelGGL(cycles_kernel, dim3(blocks_c), dim3(kBlock), 0, g->stream, g->rk_off,
                      g->rk_adj, g->n, queue, ccnt, stride, g->counters + 5);
   GPM_HIP(hipGetLastError());

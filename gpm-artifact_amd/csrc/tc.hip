@@ -81,7 +81,9 @@ __global__ __launch_bounds__(kBlock) void tc_edge_kernel(
     }
   }
   cnt = wave_sum_u64(cnt);
-  if (lane_id() == 0 && cnt) atomicAdd(out, (unsigned long long)cnt);
+  // A launch with gridDim.y = R runs R independent counts, count y into
+  // out[y] (the fused repeat of gpm_tc_repeat); gridDim.y = 1 is one count.
+  if (lane_id() == 0 && cnt) atomicAdd(out + blockIdx.y, (unsigned long long)cnt);
 }
 
 // Wave per root: roots[] lists vertices with 2 <= |N+| <= kTcWaveCap.
@@ -365,6 +367,51 @@ int ensure_tc_graph(gpm_graph* g) {
   return GPM_OK;
 }
 
+// reps edge-parallel counts as launches of kTcGraphReps counts each: one
+// kernel whose grid y index picks the count (and its counter slot), so the
+// counts of a launch overlap their dependent-load chains on the device.
+int tc_fused_repeat(gpm_graph* g, int reps, uint64_t* out) {
+  GPM_CHECK_ARG(g && out, "null pointer");
+  GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "triangle counting needs an ORIENTED graph");
+  GPM_CHECK_ARG(reps >= 1, "reps must be >= 1 (got %d)", reps);
+  int st;
+  if ((st = ensure_src(g)) != GPM_OK) return st;
+  const int G = edge_group_size(g->n, g->m);
+  const int cus = device_cus(g);
+  const uint64_t blocks = std::max<uint64_t>(
+      1, std::min<uint64_t>((std::max<uint64_t>(g->m, 1) * G + kBlock - 1) / kBlock,
+                            (uint64_t)cus * 32));
+  unsigned long long h[kTcGraphReps] = {};
+  if ((st = timing_begin(g)) != GPM_OK) return st;
+  int done = 0;
+  while (done < reps) {
+    const int batch = std::min(kTcGraphReps, reps - done);
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, g->stream, g->counters, batch);
+    dim3 grid((unsigned)blocks, (unsigned)batch), block(kBlock);
+    switch (G) {
+      case 4: hipLaunchKernelGGL(tc_edge_kernel<4>, grid, block, 0, g->stream, g->off, g->adj, g->src, g->m, g->counters); break;
+      case 8: hipLaunchKernelGGL(tc_edge_kernel<8>, grid, block, 0, g->stream, g->off, g->adj, g->src, g->m, g->counters); break;
+      case 16: hipLaunchKernelGGL(tc_edge_kernel<16>, grid, block, 0, g->stream, g->off, g->adj, g->src, g->m, g->counters); break;
+      case 32: hipLaunchKernelGGL(tc_edge_kernel<32>, grid, block, 0, g->stream, g->off, g->adj, g->src, g->m, g->counters); break;
+      default: hipLaunchKernelGGL(tc_edge_kernel<64>, grid, block, 0, g->stream, g->off, g->adj, g->src, g->m, g->counters); break;
+    }
+    GPM_HIP(hipGetLastError());
+    if (done == 0) timing_mark(g, "tc_edge_fused");
+    done += batch;
+    if (done >= reps) {
+      GPM_HIP(hipMemcpyAsync(h, g->counters, batch * sizeof(h[0]), hipMemcpyDeviceToHost,
+                             g->stream));
+      if ((st = timing_end(g)) != GPM_OK) return st;
+      GPM_HIP(hipStreamSynchronize(g->stream));
+      for (int r = 1; r < batch; ++r)
+        if (h[r] != h[0]) return fail(GPM_EHIP, "repeated triangle counts disagree");
+    }
+  }
+  g->last_ms /= (double)reps;
+  *out = h[0];
+  return GPM_OK;
+}
+
 int tc_run(gpm_graph* g, int variant, int reps, uint64_t* out) {
   GPM_CHECK_ARG(g && out, "null pointer");
   GPM_CHECK_ARG(g->flags & GPM_GRAPH_ORIENTED, "triangle counting needs an ORIENTED graph");
@@ -488,6 +535,13 @@ int gpm_tc(gpm_graph* g, int variant, uint64_t* out) {
   return ::gpm::guarded([&]() -> int {
     clear_error();
     return tc_run(g, variant, 1, out);
+  });
+}
+
+int gpm_tc_repeat_fused(gpm_graph* g, int reps, uint64_t* out) {
+  return ::gpm::guarded([&]() -> int {
+    clear_error();
+    return tc_fused_repeat(g, reps, out);
   });
 }
 

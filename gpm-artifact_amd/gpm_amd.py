@@ -335,6 +335,12 @@ class DeviceGraph:
         _check(lib().gpm_tc_repeat(self._h, C.c_int(variant), C.c_int(reps), C.byref(out)))
         return out.value
 
+    def tc_repeat_fused(self, reps: int) -> int:
+        """reps counts, 32 per launch (grid y); last_kernel_ms() is the mean per count."""
+        out = C.c_uint64()
+        _check(lib().gpm_tc_repeat_fused(self._h, C.c_int(reps), C.byref(out)))
+        return out.value
+
     def clique(self, k: int, policy: int = SCHED_DEGREE, chunk: int = 0,
                root_begin: int = 0, root_end: int = 0,
                cost: Optional[np.ndarray] = None, wave_cap: int = 0,

@@ -39,7 +39,7 @@ def queries_for(g, labels, seed):
     return qs
 
 
-def collect(scale, seed, max_rows=60000, log=print):
+def collect(scale, seed, max_rows=8000, log=print):
     g, labels = W.labeled_graph(scale, seed=seed)
     feats, works = [], []
     rng = np.random.default_rng(seed)
@@ -65,7 +65,7 @@ def spearman(a, b):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--scales", type=int, nargs="+", default=[12, 13, 14])
+    ap.add_argument("--scales", type=int, nargs="+", default=[11, 12, 13])
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
@@ -73,8 +73,8 @@ def main():
     for scale in a.scales:
         for seed in TRAIN_SEEDS:
             t = time.time()
-            f, w = collect(scale, seed, log=lambda s: print(s, f"({time.time() - t:.1f}s)",
-                                                           flush=True))
+            f, w = collect(scale, seed, log=lambda s, t=t: print(s, f"({time.time() - t:.1f}s)",
+                                                                flush=True))
             feats.append(f)
             works.append(w)
     feat = np.concatenate(feats)

@@ -255,6 +255,13 @@ int gpm_tc(gpm_graph* g, int variant, uint64_t* out);
  * in the reference.
  */
 int gpm_tc_repeat(gpm_graph* g, int variant, int reps, uint64_t* out);
+/*
+ * The same measurement for the edge-parallel count with up to 32 counts per
+ * launch (one kernel, grid y = count index, each count into its own counter
+ * slot), so independent counts overlap on the device without a graph of
+ * launches.  No counterpart in the reference.
+ */
+int gpm_tc_repeat_fused(gpm_graph* g, int reps, uint64_t* out);
 
 /*
  * Edge-parallel triangle count (the GPM_TC_EDGE kernel of gpm_tc) that

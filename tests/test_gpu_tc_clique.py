@@ -25,6 +25,9 @@ def test_tc_er_workload(gpm, orc):
         assert dg.tc(gpm.TC_EDGE) == want
         assert dg.tc(gpm.TC_VERTEX) == want
         assert dg.clique(3) == want
+        for reps in (1, 31, 32, 33, 100):
+            assert dg.tc_repeat_fused(reps) == want
+            assert dg.tc_repeat(gpm.TC_EDGE, reps) == want
 
 
 @pytest.mark.parametrize("n", [3, 4, 40, 65, 300, 700])
