@@ -429,6 +429,8 @@ int gpm_match_batch(gpm_graph* g, const gpm_query* queries, int n_queries,
  * root candidate v, 0 for other vertices (n entries); feat[4v .. 4v+3] = the
  * model's features of root candidate v (degree, depth-1 label-slice length,
  * degrees summed over that slice, query size), zeros elsewhere (4n).
+ * Both currently return GPM_EUNSUPPORTED: the root log stopped responding
+ * once on the device and stays switched off until the cause is found.
  */
 int gpm_match_root_work(gpm_graph* g, const gpm_query* query, int policy, uint64_t* work);
 int gpm_match_root_features(gpm_graph* g, const gpm_query* query, int policy, float* feat);
