@@ -38,18 +38,76 @@ METRIC = "TC/k-clique edges/sec, matches/sec; learned-vs-plain sched speedup, ex
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "clique_k4k5_rmat20_hbm_traffic.md")
+LDS_PROFILE = os.path.join(ROOT, "profiles", "clique_k4k5_rmat20_lds.md")
+# LDS peak of one MI355X: 256 CUs x 128 bytes per clock per CU (the LDS width
+# CDNA documents: 32 banks x 4 bytes per clock) x 2.4 GHz peak engine clock.
+LDS_PEAK_GBS = 256 * 128 * 2.4
+# VALU issue peak: 256 CUs x 4 SIMDs, each issuing one wave64 VALU instruction
+# per 4 clocks (16 lanes per SIMD), at 2.4 GHz -- in G instructions/s.
+VALU_PEAK_GINSTR = 1024 * 2.4 / 4
+
+
+def _profile_json(path, tag, kernel, scale):
+    """Per-launch figure of `kernel` from a committed PMC summary line
+    '<tag>: {json}', for the scale-20 workload the summaries were taken on."""
+    if scale != 20 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith(tag + ":"):
+                return json.loads(line.split(":", 1)[1]).get(kernel)
+    return None
 
 
 def measured_traffic(kernel, scale):
-    """HBM bytes per launch of `kernel` from the committed PMC summary (FETCH_SIZE +
-    WRITE_SIZE, copy-calibrated), for the scale-20 workload it was taken on."""
-    if scale != 20 or not os.path.exists(TRAFFIC_PROFILE):
-        return None
-    with open(TRAFFIC_PROFILE) as f:
-        for line in f:
-            if line.startswith("traffic_json:"):
-                return json.loads(line.split(":", 1)[1]).get(kernel)
-    return None
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    return _profile_json(TRAFFIC_PROFILE, "traffic_json", kernel, scale)
+
+
+def measured_lds(kernel, scale):
+    """{"bytes": LDS bytes loaded + stored per launch (SQ_INSTS_LDS_LOAD_BANDWIDTH +
+    SQ_INSTS_LDS_STORE_BANDWIDTH, 64-byte units), "active": LDS-active share of
+    CU cycles (SQ_LDS_IDX_ACTIVE / (GRBM_GUI_ACTIVE x CUs)), "conflict": bank-conflict
+    share of LDS-active cycles, "valu_instr": VALU wave64 instructions issued
+    (SQ_INSTS_VALU), "valu_active": VALU-busy share of SIMD cycles} per launch."""
+    return _profile_json(LDS_PROFILE, "lds_json", kernel, scale)
+
+
+def roofline(dom, dom_ms, alg_bytes, traffic, lds):
+    """Roofline of the dominant k-clique kernel.  Its rows live in LDS and its
+    inner loops are row-word ANDs and popcounts; the PMC summary shows it
+    issuing VALU instructions at ~93 % of the chip's rate with LDS at ~15 % of
+    its bandwidth, so the bound is VALU issue: achieved = VALU wave64
+    instructions per launch (SQ_INSTS_VALU) / the launch's event-timed
+    duration, peak = 1024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction.
+    LDS bandwidth and HBM traffic are reported beside it, and the canonical
+    edge-rooted plan's bytes over the same time as a plain rate (the bitmap
+    rows read each list once per root, not once per set operation, so that
+    rate exceeds any memory peak and is not a roofline)."""
+    sec = dom_ms * 1e-3
+    out = {"kernel": dom, "mean_launch_ms": dom_ms, "traffic": traffic,
+           "hbm_traffic_frac": (traffic / sec / (HBM_PEAK_GBS * 1e9)) if traffic else None,
+           "plan_bytes_per_launch": alg_bytes, "plan_bytes_rate_GBs": alg_bytes / sec / 1e9}
+    if lds and lds.get("valu_instr"):
+        achieved = lds["valu_instr"] / sec / 1e9
+        lds_rate = lds["bytes"] / sec / 1e9
+        out.update({"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_GINSTR,
+                    "unit": "G wave64 VALU instr/s", "frac": achieved / VALU_PEAK_GINSTR,
+                    "valu_instr_per_launch": lds["valu_instr"],
+                    "peak_basis": "1024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction",
+                    "lds": {"bytes_per_launch": lds["bytes"], "achieved_GBs": lds_rate,
+                            "peak_GBs": LDS_PEAK_GBS, "frac": lds_rate / LDS_PEAK_GBS,
+                            "active_frac": lds.get("active"),
+                            "bank_conflict_frac": lds.get("conflict"),
+                            "peak_basis": "256 CUs x 128 B/clk x 2.4 GHz"}})
+    elif traffic:
+        achieved = traffic / sec / 1e9
+        out.update({"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS})
+    else:
+        out.update({"bound": "valu", "achieved": None, "peak": VALU_PEAK_GINSTR,
+                    "unit": "G wave64 VALU instr/s", "frac": None})
+    return out
 
 
 def tier_of(deg, k, caps):
@@ -188,16 +246,24 @@ def extra_sched(scale):
         out["model_seconds"] = time.perf_counter() - t0
         plain = dag.clique(5)
         learned = dag.clique(5, policy=G.SCHED_LEARNED, cost=cost)
-        t_plain, t_learned = [], []
+        split = dag.clique(5, policy=G.SCHED_DEGREE_SPLIT)
+        t_plain, t_learned, t_split = [], [], []
         for _ in range(2):
             assert dag.clique(5) == plain
             t_plain.append(dag.last_kernel_ms())
+            assert dag.clique(5, policy=G.SCHED_DEGREE_SPLIT) == split
+            t_split.append(dag.last_kernel_ms())
             assert dag.clique(5, policy=G.SCHED_LEARNED, cost=cost) == learned
             t_learned.append(dag.last_kernel_ms())
         out["k5"] = {"count_plain": int(plain), "count_learned": int(learned),
-                     "identical": bool(plain == learned),
+                     "count_degree_split": int(split),
+                     "identical": bool(plain == learned == split),
                      "plain_ms": min(t_plain), "learned_ms": min(t_learned),
+                     "degree_split_ms": min(t_split),
                      "speedup": min(t_plain) / min(t_learned),
+                     # The model's own share: the same splitting driven by the
+                     # degree estimate d+^2 instead of the MLP.
+                     "speedup_vs_degree_split": min(t_split) / min(t_learned),
                      "learned_breakdown_ms": dict(dag.kernel_breakdown())}
         tri = dag.clique(3)
         deg = g.degrees().astype(np.int64)
@@ -207,7 +273,42 @@ def extra_sched(scale):
                          "motif4_induced": dict(zip(["3-star", "4-path", "tailed-triangle",
                                                      "4-cycle", "diamond", "4-clique"],
                                                     [int(x) for x in motif])),
-                         "motif4_ms": sym.last_kernel_ms()}
+                         "motif4_ms": sym.last_kernel_ms(),
+                         "motif4_breakdown_ms": dict(sym.kernel_breakdown())}
+    return out
+
+
+def extra_patterns(scale_both, scale_counting):
+    """P1-P8 (G2MinerSolver and PatternSolver): both formulations on R-MAT
+    scale_both (counts must agree), the counting formulation and its BLOCK_OPT
+    prefilter on scale_counting, where the direct enumeration of P1-P3 runs
+    for minutes."""
+    out = {"workload": f"P1-P8 on R-MAT (edge factor 16, seed 20240522): both formulations at "
+                       f"scale {scale_both}, counting formulation at scale {scale_counting}"}
+    pids = ["P1", "P2", "P3", "P4", "P5", "P6", "P7", "P8"]
+    g = G.rmat_graph(scale_both, 16, 20240522)
+    rows = {}
+    with G.DeviceGraph(g, oriented=False) as sym:
+        for pid in pids:
+            _, direct = sym.pattern(pid, G.FORM_G2MINER)
+            t_direct = sym.last_kernel_ms()
+            _, counted = sym.pattern(pid, G.FORM_PATTERN)
+            t_count = sym.last_kernel_ms()
+            rows[pid] = {"count": int(direct), "identical": int(direct) == int(counted),
+                         "g2miner_ms": t_direct, "pattern_ms": t_count}
+    out[f"scale{scale_both}"] = rows
+    print("[bench] patterns small scale done", file=sys.stderr, flush=True)
+    g = G.rmat_graph(scale_counting, 16, 20240522)
+    rows = {}
+    with G.DeviceGraph(g, oriented=False) as sym:
+        for pid in pids:
+            _, counted = sym.pattern(pid, G.FORM_PATTERN)
+            t_count = sym.last_kernel_ms()
+            _, pref = sym.pattern(pid, G.FORM_BLOCK_PREFILTER)
+            rows[pid] = {"count": int(counted), "identical": int(counted) == int(pref),
+                         "pattern_ms": t_count, "block_prefilter_ms": sym.last_kernel_ms()}
+    out[f"scale{scale_counting}"] = rows
+    print("[bench] patterns done", file=sys.stderr, flush=True)
     return out
 
 
@@ -280,15 +381,25 @@ def extra_match(scale, big_scale):
                        f"R-MAT scale {big_scale} (8 labels)"}
     with G.DeviceGraph(g, oriented=False, labels=labels, n_labels=W.N_LABELS) as dg:
         counts = {}
-        for pol, pname in ((G.SCHED_DEGREE, "plain"), (G.SCHED_LEARNED, "learned")):
+        # plain / learned: the autotuner picks each query's probe variant;
+        # *_binary force per-lane binary search, *_merge the chunked merge.
+        for pname, kw in (("plain", dict(policy=G.SCHED_DEGREE)),
+                          ("learned", dict(policy=G.SCHED_LEARNED)),
+                          ("plain_binary", dict(policy=G.SCHED_DEGREE,
+                                                set_variant=G.MATCH_SETV_BINARY)),
+                          ("plain_merge", dict(policy=G.SCHED_DEGREE,
+                                               set_variant=G.MATCH_SETV_MERGE))):
             t0 = time.perf_counter()
-            c = dg.match_batch([named[q] for q in four], autotune=1, policy=pol)
+            c = dg.match_batch([named[q] for q in four], autotune=1, **kw)
             first = (time.perf_counter() - t0) * 1e3
             counts[pname] = [int(x) for x in c]
             res[pname] = {"ms": dg.last_kernel_ms(), "first_call_ms": first}
+            print(f"[bench] large named batch {pname} {res[pname]['ms']:.1f} ms",
+                  file=sys.stderr, flush=True)
         res["counts"] = counts["plain"]
-        res["identical"] = counts["plain"] == counts["learned"]
+        res["identical"] = all(c == counts["plain"] for c in counts.values())
         res["learned_speedup"] = res["plain"]["ms"] / res["learned"]["ms"]
+        res["merge_vs_binary_speedup"] = res["plain_binary"]["ms"] / res["plain_merge"]["ms"]
     out["named_batch_large"] = res
     print("[bench] large named batch done", file=sys.stderr, flush=True)
     # Uneven batch: seeded random connected queries (random spanning tree plus
@@ -338,8 +449,8 @@ def main():
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--extras", default="tc,sched,match",
-                    help="comma list of extra workloads: tc, sched, match (or none)")
+    ap.add_argument("--extras", default="tc,sched,match,patterns",
+                    help="comma list of extra workloads: tc, sched, match, patterns (or none)")
     ap.add_argument("--sched-scale", type=int, default=22)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (to scale "
@@ -440,21 +551,7 @@ def main():
         "matches_per_s": (counts[4] + counts[5]) / (call_ms * 1e-3 / args.steps),
         "separate_passes_ms": separate,
         "kernel_ms_per_step": {k: v / args.steps for k, v in sorted(kern_ms.items())},
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            # Measured HBM bytes over the launch time, as a fraction of the
-            # peak: how far the kernel is from being HBM-bound.
-            "traffic_frac": (traffic / (dom_ms * 1e-3) / (HBM_PEAK_GBS * 1e9)
-                             if traffic and dom_ms else None),
-            "algorithmic_bytes_per_launch": alg_bytes,
-            "mean_launch_ms": dom_ms,
-        },
+        "roofline": roofline(dom, dom_ms, alg_bytes, traffic, measured_lds(dom, args.scale)),
         "plan_bytes": {f"k{k}": int(v) for k, v in plan_total.items()},
         "prep_seconds": round(t_prep, 3),
     }
@@ -471,6 +568,8 @@ def main():
             out["extra"]["sched"] = extra_sched(args.sched_scale)
         elif e == "match":
             out["extra"]["match"] = extra_match(args.match_scale, args.sched_scale)
+        elif e == "patterns":
+            out["extra"]["patterns"] = extra_patterns(12, 16)
         print(f"[bench] extra {e} done in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
               flush=True)
     print(json.dumps(out), flush=True)

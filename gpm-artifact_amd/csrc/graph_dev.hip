@@ -141,8 +141,11 @@ int ensure_scratch(gpm_graph* g, size_t bytes) {
     g->scratch_bytes = 0;
   }
   size_t want = std::max(bytes, (size_t)1 << 20);
-  if (hipMalloc(&g->scratch, want) != hipSuccess)
+  if (hipMalloc(&g->scratch, want) != hipSuccess) {
+    g->scratch = nullptr;
+    (void)hipGetLastError();  // clear the sticky error: callers may retry smaller
     return fail(GPM_ENOMEM, "device scratch of %zu bytes", want);
+  }
   g->scratch_bytes = want;
   return GPM_OK;
 }

@@ -66,6 +66,7 @@ struct QPlan {
   uint32_t chunk;           // roots per task
   uint32_t task_off;        // first task of this query
   uint32_t qid;             // index of the query in the caller's array
+  uint32_t setv;            // membership probes: kSetBinary or kSetMerge
   // Neighbour-label filter per depth: a data vertex matched at depth d needs
   // at least nlf_count[d][k] neighbours of label nlf_label[d][k].
   uint32_t nlf_n[kMaxQ];
@@ -182,6 +183,9 @@ struct WaveState {
   uint32_t np[kMaxQ];
   uint32_t plo[kMaxQ][kMaxQ - 1];
   uint32_t plen[kMaxQ][kMaxQ - 1];
+  // MERGE probes: per probed slice, a position before which every entry is
+  // below the rest of the anchor stream (advanced window by window).
+  uint32_t pcur[kMaxQ][kMaxQ - 1];
 };
 
 // Sets up depth d: picks the backward neighbour with the shortest label slice.
@@ -217,6 +221,7 @@ __device__ __forceinline__ void init_depth(const DevGraph& g, const QPlan& p, Wa
     s.plen[d][at] = len;
     s.plo[d][at] = lo;
   }
+  for (uint32_t k = 0; k < np; ++k) s.pcur[d][k] = 0u;
   s.np[d] = np;
 }
 
@@ -237,41 +242,172 @@ __device__ __forceinline__ bool nlf_ok(const DevGraph& g, const QPlan& p, uint32
   return true;
 }
 
-// Does slice entry x pass the filters of depth d?
-__device__ __forceinline__ bool passes(const DevGraph& g, const QPlan& p, const WaveState& s,
-                                       uint32_t d, uint32_t x) {
-  uint32_t same = p.same[d];
-  while (same) {
-    const uint32_t j = __builtin_ctz(same);
-    same &= same - 1;
-    if (s.emb[j] == x) return false;
+// Membership probes of a window.  BINARY: every lane binary-searches its
+// entry in the probed slice (log2 |P| dependent loads per lane, 64 distinct
+// addresses per step).  MERGE: the wave walks the probed slice once per
+// window in 64-entry chunks (one coalesced load each) and every lane finds
+// its entry among the chunk's values held by the other lanes.
+constexpr uint32_t kSetBinary = 0, kSetMerge = 1;
+// MERGE walks at most this many chunks per window; entries still unresolved
+// then binary-search the rest of the slice (a cap: the choice in probe()
+// keeps the expected walk shorter).
+constexpr uint32_t kMergeChunks = 4;
+
+// First index i in [lo, hi) with a[i] >= key, else hi; every argument is
+// the same in all lanes, so the loads are wave-uniform.
+__device__ __forceinline__ uint32_t lower_index_uniform(const uint32_t* __restrict__ a, uint32_t lo,
+                                                        uint32_t hi, uint32_t key) {
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (uniform(a[mid]) < key)
+      lo = mid + 1;
+    else
+      hi = mid;
   }
-  if (!nlf_ok(g, p, d, x)) return false;
-  for (uint32_t k = 0; k < s.np[d]; ++k)
-    if (!sorted_contains(g.ladj + s.plo[d][k], s.plen[d][k], x)) return false;
-  return true;
+  return lo;
 }
 
-// Filters of tail depth d for an entry x (label already right): degree,
-// NLF, injectivity against the matched depths below the tail, and
-// membership in the lists of d's backward neighbours -- all of them when
-// `full`, all but the streamed anchor otherwise.
-__device__ __forceinline__ bool passes_tail(const DevGraph& g, const QPlan& p, const WaveState& s,
-                                            uint32_t d, uint32_t x, bool full) {
-  for (uint32_t same = p.same[d] & ((1u << p.tail) - 1u); same; same &= same - 1)
-    if (s.emb[__builtin_ctz(same)] == x) return false;
-  if (!nlf_ok(g, p, d, x)) return false;
+// First index i in [c, len) with a[i] >= key (len if none), searched from
+// c by doubling steps: a walk that resumes near its last position costs one
+// or two loads.  Wave-uniform arguments.
+__device__ __forceinline__ uint32_t lower_index_gallop(const uint32_t* __restrict__ a, uint32_t c,
+                                                       uint32_t len, uint32_t key) {
+  if (c >= len || uniform(a[c]) >= key) return c;
+  uint32_t lo = c + 1, step = 1, hi;
+  while (true) {
+    hi = c + step;
+    if (hi >= len) {
+      hi = len;
+      break;
+    }
+    if (uniform(a[hi]) >= key) break;
+    lo = hi + 1;
+    step <<= 1;
+  }
+  return lower_index_uniform(a, lo, hi, key);
+}
+
+// Is this lane's x in the sorted slice a[0, len)?  Asked for the lanes with
+// `live` set, whose x must increase with the lane index (a window of a
+// sorted slice); the others get false.  Whole-wave call: every lane must
+// reach it with the same a, len and cursor.  *cursor (LDS, may be null)
+// holds a position before which every entry is below the live x; it is
+// advanced past the entries <= the largest live x, for the next window of
+// the same stream.  The walk starts at the first entry >= the smallest
+// live x; after each chunk the lanes whose x is <= the chunk's last value
+// are resolved (x lies in the chunk or in no chunk), and the walk jumps to
+// the first entry >= the smallest unresolved x.
+__device__ __noinline__ bool member_merge(const uint32_t* __restrict__ a, uint32_t len, uint32_t x, bool live,
+                             uint32_t* cursor) {
+  const uint32_t lane = lane_id();
+  const uint64_t live_mask = wave_ballot(live);
+  uint64_t open = live_mask;
+  bool found = false;
+  if (open == 0ull || len == 0u) return false;
+  const uint32_t xmax =
+      uniform(__builtin_amdgcn_readlane(x, (int)uniform(63 - __builtin_clzll(live_mask))));
+  uint32_t pos = lower_index_gallop(
+      a, cursor ? uniform(*cursor) : 0u, len,
+      uniform(__builtin_amdgcn_readlane(x, (int)uniform(__builtin_ctzll(open)))));
+  uint32_t next = pos;  // cursor for the next window
+  for (uint32_t round = 0; pos < len; ++round) {
+    if (round == kMergeChunks) {
+      if ((open >> lane) & 1ull) found = sorted_contains(a + pos, len - pos, x);
+      break;
+    }
+    const uint32_t i = pos + lane;
+    const uint32_t v = i < len ? a[i] : 0xFFFFFFFFu;  // ids are < 2^32 - 1
+    const uint32_t vmax = uniform(__builtin_amdgcn_readlane(v, kWave - 1));
+    // Largest j with v[j] <= x (0 when none): six steps over the lanes.
+    uint32_t j = 0;
+#pragma unroll
+    for (uint32_t half = 32; half > 0; half >>= 1)
+      if ((uint32_t)__shfl((int)v, (int)(j + half), kWave) <= x) j += half;
+    const bool hit = (uint32_t)__shfl((int)v, (int)j, kWave) == x;
+    const bool mine = ((open >> lane) & 1ull) && x <= vmax;
+    if (mine) found = hit;
+    next = pos + (uint32_t)__builtin_popcountll(wave_ballot(i < len && v <= xmax));
+    open &= ~wave_ballot(mine);
+    if (open == 0ull || vmax == 0xFFFFFFFFu) break;
+    pos = lower_index_gallop(
+        a, pos + kWave, len,
+        uniform(__builtin_amdgcn_readlane(x, (int)uniform(__builtin_ctzll(open)))));
+    next = pos;
+  }
+  if (cursor) *cursor = next;
+  return found;
+}
+
+// One membership probe of a window streamed from an anchor slice of alen
+// entries, by the plan's variant.  MERGE pays one gallop from the stream's
+// cursor (a full lower-bound search on the first window) and about
+// c = len / alen chunk loads per window, against log2(len) + 1 dependent
+// loads for every lane's binary search; the first search only pays off
+// over many windows, so a probe uses MERGE where the stream has at least
+// kMergeMinWindows windows and 2c + 3 <= floor(log2 len), and binary search
+// otherwise.  Without a cursor (probes re-derived per window) it always
+// binary-searches.  Whole-wave call.
+constexpr uint32_t kMergeMinWindows = 4;
+template <bool kMerge>
+__device__ __forceinline__ bool probe(const DevGraph& g, uint32_t setv, uint32_t lo, uint32_t len,
+                                      uint32_t alen, uint32_t x, bool ok, uint32_t* cursor) {
+  if (kMerge && setv == kSetMerge && cursor && alen >= kMergeMinWindows * kWave && len > 1u) {
+    const uint32_t c = (len + alen - 1) / alen;
+    if (2 * c + 3 <= (uint32_t)(31 - __builtin_clz(len)))
+      return member_merge(g.ladj + lo, len, x, ok, cursor);
+  }
+  return ok && sorted_contains(g.ladj + lo, len, x);
+}
+
+// Filters of depth d for this lane's slice entry x (live: the lane holds an
+// entry of the window): injectivity, degree and NLF, and membership in the
+// other backward neighbours' label slices.  Whole-wave call.  BINARY checks
+// degree and NLF first (one load, before the searches); MERGE probes first,
+// so only the entries in every slice pay the signature load.
+template <bool kMerge>
+__device__ __forceinline__ bool window_passes(const DevGraph& g, const QPlan& p, WaveState& s,
+                                              uint32_t d, uint32_t x, bool live) {
+  bool ok = live;
+  for (uint32_t same = p.same[d]; ok && same; same &= same - 1)
+    if (s.emb[__builtin_ctz(same)] == x) ok = false;
+  const bool merge = kMerge && p.setv == kSetMerge;
+  if (!merge && ok) ok = nlf_ok(g, p, d, x);
+  const uint32_t np = uniform(s.np[d]), alen = uniform(s.sl[d]);
+  for (uint32_t k = 0; k < np; ++k)
+    ok = probe<kMerge>(g, p.setv, uniform(s.plo[d][k]), uniform(s.plen[d][k]), alen, x, ok, &s.pcur[d][k]);
+  if (merge && ok) ok = nlf_ok(g, p, d, x);
+  return ok;
+}
+
+// Filters of tail depth d for this lane's entry x (label already right) of
+// a window streamed from an anchor slice of alen entries: degree, NLF,
+// injectivity against the matched depths below the tail, and membership in
+// the lists of d's backward neighbours -- all of them when `full`, all but
+// the streamed anchor otherwise.  Whole-wave call; the live lanes' x
+// increase with the lane index.
+template <bool kMerge>
+__device__ __forceinline__ bool window_passes_tail(const DevGraph& g, const QPlan& p,
+                                                   WaveState& s, uint32_t d, uint32_t x,
+                                                   bool live, bool full, uint32_t alen) {
+  bool ok = live;
+  for (uint32_t same = p.same[d] & ((1u << p.tail) - 1u); ok && same; same &= same - 1)
+    if (s.emb[__builtin_ctz(same)] == x) ok = false;
+  const bool merge = kMerge && p.setv == kSetMerge;
+  if (!merge && ok) ok = nlf_ok(g, p, d, x);
   if (!full) {
-    for (uint32_t k = 0; k < s.np[d]; ++k)
-      if (!sorted_contains(g.ladj + s.plo[d][k], s.plen[d][k], x)) return false;
-    return true;
+    const uint32_t np = uniform(s.np[d]);
+    for (uint32_t k = 0; k < np; ++k)
+      ok = probe<kMerge>(g, p.setv, uniform(s.plo[d][k]), uniform(s.plen[d][k]), alen, x, ok,
+                 &s.pcur[d][k]);
+  } else {
+    for (uint32_t bn = p.bn[d]; bn; bn &= bn - 1) {
+      uint32_t lo, len;
+      label_slice(g, s.emb[__builtin_ctz(bn)], p.label[d], &lo, &len);
+      ok = probe<kMerge>(g, p.setv, uniform(lo), uniform(len), alen, x, ok, nullptr);
+    }
   }
-  for (uint32_t bn = p.bn[d]; bn; bn &= bn - 1) {
-    uint32_t lo, len;
-    label_slice(g, s.emb[__builtin_ctz(bn)], p.label[d], &lo, &len);
-    if (!sorted_contains(g.ladj + lo, len, x)) return false;
-  }
-  return true;
+  if (merge && ok) ok = nlf_ok(g, p, d, x);
+  return ok;
 }
 
 // A trial task stops once its weighted steps pass this budget: a plan with
@@ -307,7 +443,7 @@ __device__ uint64_t window_cost(const DevGraph& g, const QPlan& p, const WaveSta
 // |A_B| for a subset B is counted while streaming the candidates of B's
 // first member (each x tested against the later members' full filters).
 // Arithmetic is modulo 2^64, exact when the true count fits.
-template <int kMode>
+template <int kMode, bool kMerge>
 __device__ uint64_t count_tail(const DevGraph& g, const QPlan& p, WaveState& s, uint64_t& steps) {
   const uint32_t lane = lane_id();
   uint64_t prod = 1;
@@ -327,15 +463,12 @@ __device__ uint64_t count_tail(const DevGraph& g, const QPlan& p, WaveState& s, 
         if (kMode == kModeTrial && steps >= kTrialBudget) break;
         steps += wcost;
         const uint32_t i = cur + lane;
-        uint32_t mask = 0;
-        if (i < sl) {
-          const uint32_t x = g.ladj[sb + i];
-          if (passes_tail(g, p, s, d, x, false)) {
-            mask = 1u << m;
-            for (uint32_t m2 = m + 1; m2 < k; ++m2)
-              if (passes_tail(g, p, s, p.grp_dep[gi][m2], x, true)) mask |= 1u << m2;
-          }
-        }
+        const bool live = i < sl;
+        const uint32_t x = live ? g.ladj[sb + i] : 0u;
+        const bool ok = window_passes_tail<kMerge>(g, p, s, d, x, live, false, sl);
+        uint32_t mask = ok ? 1u << m : 0u;
+        for (uint32_t m2 = m + 1; m2 < k; ++m2)
+          if (window_passes_tail<kMerge>(g, p, s, p.grp_dep[gi][m2], x, ok, true, sl)) mask |= 1u << m2;
         // Subsets B with lowest member m and B within mask.
         for (uint32_t B = 1u << m; B < 8u; ++B)
           if ((B & ((1u << m) - 1u)) == 0 && (B & ~mask) == 0 && (B >> m) & 1u) a[B] += 1;
@@ -357,15 +490,19 @@ __device__ uint64_t count_tail(const DevGraph& g, const QPlan& p, WaveState& s, 
 
 // Counts the embeddings of plan p rooted at v0 whose depth-1 vertex is one
 // of the entries [lo1, hi1) of the depth-1 slice (all: 0, ~0u): a DFS over
-// depths 1..tail-1, each leaf of which adds the tail count.
-template <int kMode>
+// depths 1..tail-1, each leaf of which adds the tail count.  With nparts2 >
+// 1 (plans whose DFS goes below depth 2 only) every depth-2 stream is cut
+// into nparts2 equal ranges of its anchor slice and only range part2 is
+// walked: the parts of one depth-1 entry partition its subtree.
+template <int kMode, bool kMerge>
 __device__ uint64_t count_root(const DevGraph& g, const QPlan& p, WaveState& s, uint32_t v0,
-                               uint32_t lo1, uint32_t hi1, uint64_t& steps) {
+                               uint32_t lo1, uint32_t hi1, uint32_t part2, uint32_t nparts2,
+                               uint64_t& steps) {
   const uint32_t lane = lane_id();
   const uint32_t n = p.n;
   if (n == 1) return 1;
   s.emb[0] = v0;
-  if (p.tail == 1) return count_tail<kMode>(g, p, s, steps);
+  if (p.tail == 1) return count_tail<kMode, kMerge>(g, p, s, steps);
   uint64_t total = 0;
   init_depth(g, p, s, 1);
   s.cur[1] = lo1;
@@ -382,7 +519,8 @@ __device__ uint64_t count_root(const DevGraph& g, const QPlan& p, WaveState& s, 
       }
       steps += kMode != kModeCount ? window_cost(g, p, s, dd, false) : 1u;
       const uint32_t i = cur + lane;
-      const bool ok = i < s.sl[dd] && passes(g, p, s, dd, g.ladj[s.sb[dd] + i]);
+      const bool live = i < s.sl[dd];
+      const bool ok = window_passes<kMerge>(g, p, s, dd, live ? g.ladj[s.sb[dd] + i] : 0u, live);
       s.mask[dd] = wave_ballot(ok);
       s.base[dd] = cur;
       s.cur[dd] = cur + kWave;
@@ -393,9 +531,14 @@ __device__ uint64_t count_root(const DevGraph& g, const QPlan& p, WaveState& s, 
     s.mask[dd] = m & (m - 1);
     s.emb[dd] = uniform(g.ladj[s.sb[dd] + s.base[dd] + j]);
     if (dd + 1 == p.tail) {
-      total += count_tail<kMode>(g, p, s, steps);
+      total += count_tail<kMode, kMerge>(g, p, s, steps);
     } else {
       init_depth(g, p, s, dd + 1);
+      if (dd == 1 && nparts2 > 1) {
+        const uint64_t sl2 = s.sl[2];
+        s.cur[2] = (uint32_t)(sl2 * part2 / nparts2);
+        s.sl[2] = (uint32_t)(sl2 * (part2 + 1) / nparts2);
+      }
       steps += kMode != kModeCount ? 2u * __builtin_popcount(p.bn[dd + 1]) : 1u;
       ++d;
     }
@@ -419,6 +562,7 @@ __device__ __forceinline__ uint32_t find_plan(const QPlan* plans, uint32_t nq, u
 // into the plan's root list) of plan `plan`.
 struct HeavyTask {
   uint32_t plan, root, lo, hi;
+  uint32_t part2, nparts2;  // depth-2 range of the task (0, 1: all)
 };
 
 // Persistent enumeration: every wave pulls tasks from one queue until it is
@@ -429,7 +573,9 @@ struct HeavyTask {
 // task past kTrialBudget (its counts are then partial and unused), and
 // adds, per plan, the total to work[3 qid], the largest task's to
 // work[3 qid + 1] and the number of stopped tasks to work[3 qid + 2].
-template <int kMode>
+// kMerge: the batch has MERGE plans (a launch without them is built without
+// the merge walk, which keeps its register count at 6 waves per SIMD).
+template <int kMode, bool kMerge>
 __global__ __launch_bounds__(kBlock) void match_kernel(
     DevGraph g, const QPlan* __restrict__ plans, uint32_t nq, uint32_t n_tasks,
     const uint32_t* __restrict__ roots, const uint32_t* __restrict__ rlen,
@@ -451,8 +597,9 @@ __global__ __launch_bounds__(kBlock) void match_kernel(
       const HeavyTask h = heavy[t];
       const QPlan& p = plans[uniform(h.plan)];
       qid = p.qid;
-      c = count_root<kMode>(g, p, s, uniform(roots[p.root_off + uniform(h.root)]),
-                             uniform(h.lo), uniform(h.hi), steps);
+      c = count_root<kMode, kMerge>(g, p, s, uniform(roots[p.root_off + uniform(h.root)]),
+                                    uniform(h.lo), uniform(h.hi), uniform(h.part2),
+                                    uniform(h.nparts2), steps);
     } else {
       const uint32_t pi = uniform(find_plan(plans, nq, t - n_heavy));
       const QPlan& p = plans[pi];
@@ -461,8 +608,8 @@ __global__ __launch_bounds__(kBlock) void match_kernel(
       const uint32_t r1 = min(r0 + p.chunk, p.n_roots);
       for (uint32_t r = r0; r < r1; ++r) {
         if (p.tail > 1 && uniform(rlen[p.root_off + r]) > split_len) continue;  // a heavy task
-        c += count_root<kMode>(g, p, s, uniform(roots[p.root_off + r]), 0u, 0xFFFFFFFFu,
-                                steps);
+        c += count_root<kMode, kMerge>(g, p, s, uniform(roots[p.root_off + r]), 0u, 0xFFFFFFFFu,
+                                       0u, 1u, steps);
       }
     }
     if (lane == 0 && c) atomicAdd(&counts[qid], (unsigned long long)c);
@@ -544,6 +691,27 @@ double estimate_root_cost(const gpm_graph* g, const QPlan& p) {
   double nodes = 1.0, work = 0.0;
   for (uint32_t d = 1; d < p.n; ++d) work += estimate_depth(st, p.label, p.bn[d], p.label[d], &nodes);
   return std::max(1.0, work);
+}
+
+// Membership-probe variant of plan p: the caller's, else under the
+// autotuner MERGE when some depth with probes streams slices of expected
+// length >= kMergeMinSlice (label statistics) and BINARY otherwise.  MERGE
+// only pays on long streams (many windows per cursor); on R-MAT with 8
+// labels (expected slices of ~4 entries) it measured 15-27 % slower than
+// BINARY on every named query and batch, so short expected slices keep
+// BINARY.
+constexpr double kMergeMinSlice = 256.0;
+uint32_t pick_variant(const gpm_graph* g, const QPlan& p, const gpm_tuning& tun) {
+  if (tun.set_variant == GPM_MATCH_SETV_BINARY) return kSetBinary;
+  if (tun.set_variant == GPM_MATCH_SETV_MERGE) return kSetMerge;
+  if (!tun.autotune) return kSetBinary;
+  const LabelStats st{g};
+  for (uint32_t d = 1; d < p.n; ++d) {
+    if (__builtin_popcount(p.bn[d]) < 2) continue;
+    for (uint32_t m = p.bn[d]; m; m &= m - 1)
+      if (st.slice(p.label[__builtin_ctz(m)], p.label[d]) >= kMergeMinSlice) return kSetMerge;
+  }
+  return kSetBinary;
 }
 
 // Cost-based order from start vertex s0: the order grows greedily by the
@@ -787,6 +955,7 @@ struct MatchBatch {
   unsigned long long* d_counts = nullptr;
   unsigned long long* d_work = nullptr;  // wave steps per counter (plan trials, root logs)
   int mode = kModeCount;
+  bool merge = false;  // some plan probes with MERGE
   unsigned int* d_rootcnt = nullptr;
   unsigned int* d_queue = nullptr;
   HeavyTask* d_heavy = nullptr;  // own allocation
@@ -821,6 +990,7 @@ std::vector<uint32_t> batch_key(const gpm_query* queries, int n_queries, const g
   k.push_back((uint32_t)t.policy);
   k.push_back(t.chunk);
   k.push_back((uint32_t)t.use_graph);
+  k.push_back((uint32_t)t.set_variant);
   if (t.root_model) {
     const uint32_t* w = reinterpret_cast<const uint32_t*>(t.root_model);
     k.insert(k.end(), w, w + kMlpWeights);
@@ -844,9 +1014,11 @@ int launch_batch(gpm_graph* g, MatchBatch* b) {
   }
   hipLaunchKernelGGL(zero_kernel, dim3((b->n_queries + 64) / 64 + 1), dim3(64), 0, g->stream,
                      b->d_counts, b->n_queries, b->d_queue, 4);
-  hipLaunchKernelGGL(b->mode == kModeTrial ? match_kernel<kModeTrial>
-                                            : b->mode == kModeLog ? match_kernel<kModeLog>
-                                                                  : match_kernel<kModeCount>,
+  auto* kernel = b->mode == kModeTrial ? &match_kernel<kModeTrial, false>
+                 : b->mode == kModeLog  ? &match_kernel<kModeLog, false>
+                 : b->merge             ? &match_kernel<kModeCount, true>
+                                        : &match_kernel<kModeCount, false>;
+  hipLaunchKernelGGL(kernel,
                      dim3(b->grid),
                      dim3(kBlock), 0, g->stream, b->dg,
                      b->d_plans, b->nq, b->n_tasks, b->d_roots, b->d_rlen, b->d_heavy,
@@ -874,7 +1046,8 @@ int build_graph(MatchBatch* b) {
                    &b->d_roots, &b->d_rlen,    &b->d_heavy, &b->n_heavy,
                    &b->split_len, &b->d_queue, &b->d_counts, &b->d_work};
   hipKernelNodeParams mp{};
-  mp.func = reinterpret_cast<void*>(&match_kernel<kModeCount>);
+  mp.func = b->merge ? reinterpret_cast<void*>(&match_kernel<kModeCount, true>)
+                     : reinterpret_cast<void*>(&match_kernel<kModeCount, false>);
   mp.gridDim = dim3(b->grid);
   mp.blockDim = dim3(kBlock);
   mp.sharedMemBytes = 0;
@@ -1167,6 +1340,8 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
   }
   for (uint32_t i : perm) {
     QPlan q = plans[i];
+    q.setv = trial || log_roots ? kSetBinary : pick_variant(g, q, tun);
+    b->merge = b->merge || q.setv == kSetMerge;
     if (trial || log_roots)
       q.chunk = 1;
     else if (learned && !tun.chunk)
@@ -1206,7 +1381,7 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
         const uint32_t step = (len + np - 1) / np;
         marks[q.root_off + r] = 0xFFFFFFFFu;
         for (uint32_t lo = 0; lo < len; lo += step) {
-          heavy.push_back({li, r, lo, std::min(len, lo + step)});
+          heavy.push_back({li, r, lo, std::min(len, lo + step), 0u, 1u});
           hlen.push_back(c * (double)(std::min(len, lo + step) - lo) / (double)len);
         }
         continue;
@@ -1214,12 +1389,59 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
       if (len <= kSplitLen) continue;
       for (uint32_t lo = 0; lo < len; lo += kSplitLen) {
         if (trial && !sampled(q.qid, troots[q.root_off + r], lo)) continue;
-        heavy.push_back({li, r, lo, std::min(len, lo + kSplitLen)});
+        heavy.push_back({li, r, lo, std::min(len, lo + kSplitLen), 0u, 1u});
         hlen.push_back(len);
       }
     }
   }
-  if (learned) {
+  // Short batches (the autotuner, no model): when the tasks so far cannot
+  // keep kTasksPerWave tasks on every resident wave, the plans whose DFS goes
+  // below depth 2 are dealt out whole -- every root as tasks of a few
+  // depth-1 entries, each depth-2 stream of a task cut into nparts2 ranges
+  // -- so a few dense subtrees (small graphs, dense queries) spread over the
+  // chip.  About kRefineTasksPerWave tasks per resident wave in all.
+  constexpr uint64_t kRefineTasksPerWave = 4;
+  const uint64_t have_tasks = (uint64_t)t + heavy.size();
+  bool refined = false;
+  if (tun.autotune && !learned && !trial && !log_roots && !tun.chunk &&
+      have_tasks < kTasksPerWave * waves) {
+    uint64_t e1 = 0;  // depth-1 entries of the roots of the refined plans
+    for (uint32_t li = 0; li < nq; ++li)
+      if (laid[li].tail > 2)
+        for (uint32_t r = 0; r < laid[li].n_roots; ++r) e1 += rlen[laid[li].root_off + r];
+    const uint64_t target_tasks = kRefineTasksPerWave * waves;
+    if (e1 > 0) {
+      refined = true;
+      const uint32_t part1 = (uint32_t)std::max<uint64_t>(1, (e1 + target_tasks - 1) / target_tasks);
+      const uint32_t nparts2 =
+          part1 > 1 ? 1u : (uint32_t)std::min<uint64_t>(16, (target_tasks + e1 - 1) / e1);
+      marks.assign(std::max<uint64_t>(root_total, 1), 0u);
+      std::vector<HeavyTask> kept;
+      std::vector<double> kept_len;
+      for (size_t i = 0; i < heavy.size(); ++i)
+        if (laid[heavy[i].plan].tail <= 2) {  // parts of plans not refined stay
+          kept.push_back(heavy[i]);
+          kept_len.push_back(hlen[i]);
+          marks[laid[heavy[i].plan].root_off + heavy[i].root] = 0xFFFFFFFFu;
+        }
+      heavy.swap(kept);
+      hlen.swap(kept_len);
+      for (uint32_t li = 0; li < nq; ++li) {
+        const QPlan& q = laid[li];
+        if (q.tail <= 2) continue;
+        for (uint32_t r = 0; r < q.n_roots; ++r) {
+          const uint32_t len = rlen[q.root_off + r];
+          marks[q.root_off + r] = 0xFFFFFFFFu;  // no embeddings when len == 0
+          for (uint32_t lo = 0; lo < len; lo += part1)
+            for (uint32_t k = 0; k < nparts2; ++k) {
+              heavy.push_back({li, r, lo, std::min(len, lo + part1), k, nparts2});
+              hlen.push_back(len);
+            }
+        }
+      }
+    }
+  }
+  if (learned || refined) {
     // The kernel skips roots whose rlen exceeds split_len in the light
     // chunks: mark exactly the roots dealt out as parts.
     b->split_len = 0;
@@ -1569,6 +1791,8 @@ extern "C" int gpm_match_batch(gpm_graph* g, const gpm_query* queries, int n_que
     if (tuning) tun = *tuning;
     GPM_CHECK_ARG(tun.policy == GPM_SCHED_DEGREE || tun.policy == GPM_SCHED_LEARNED,
                   "unknown policy %d", tun.policy);
+    GPM_CHECK_ARG(tun.set_variant >= GPM_MATCH_SETV_AUTO && tun.set_variant <= GPM_MATCH_SETV_MERGE,
+                  "unknown set variant %d", tun.set_variant);
     // Autotuned batches are not captured into a hipGraph: that combination
     // stopped responding on the device with the earlier probe-based tuner and
     // stays disabled until its cause is known.  Tuned batches run as one

@@ -53,7 +53,7 @@ class Query(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("autotune", C.c_int), ("policy", C.c_int),
                 ("chunk", C.c_uint32), ("use_graph", C.c_int),
-                ("root_model", _f32p), ("reserved", C.c_uint32 * 2)]
+                ("root_model", _f32p), ("set_variant", C.c_int), ("reserved", C.c_uint32)]
 
 
 _lib = None
@@ -402,12 +402,15 @@ class DeviceGraph:
 
     def match_batch(self, queries: Sequence, autotune: int = 1,
                     policy: int = SCHED_DEGREE, chunk: int = 0,
-                    use_graph: int = 0, model: Optional[np.ndarray] = None) -> np.ndarray:
+                    use_graph: int = 0, model: Optional[np.ndarray] = None,
+                    set_variant: int = 0) -> np.ndarray:
         """queries: sequence of (n, edges [(u, v), ...], labels [..]); model: packed
-        root-cost MLP (4553 float32, see MatchModel) for the LEARNED policy."""
+        root-cost MLP (4553 float32, see MatchModel) for the LEARNED policy;
+        set_variant: MATCH_SETV_AUTO / _BINARY / _MERGE."""
         arr, keep = self._queries(queries)
         t = Tuning()
         t.autotune, t.policy, t.chunk, t.use_graph = autotune, policy, chunk, use_graph
+        t.set_variant = set_variant
         if model is not None:
             self._model = np.ascontiguousarray(model, dtype=np.float32)
             assert self._model.shape == (4553,)
@@ -444,6 +447,7 @@ def tc_edge_noalloc(d_offsets: int, d_adj: int, n: int, m: int, d_count: int,
                                      C.c_void_p(stream)))
 
 
+MATCH_SETV_AUTO, MATCH_SETV_BINARY, MATCH_SETV_MERGE = 0, 1, 2
 SETOP_ISECT_COUNT, SETOP_ISECT, SETOP_DIFF_COUNT, SETOP_DIFF, SETOP_FUSE = 0, 1, 2, 3, 4
 SETV_BINARY, SETV_TWO_PHASE, SETV_MERGE, SETV_HASH, SETV_BITMAP = 0, 1, 2, 3, 4
 

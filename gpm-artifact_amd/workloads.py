@@ -9,6 +9,9 @@
   sampled_queries(...)  connected queries of 4..10 vertices cut out of the
                         data graph by seeded random expansion (so every query
                         has at least one embedding), with the data labels
+  batch_queries()       the batched-matching workload's 256 seeded connected
+                        queries of 4..10 vertices (random_queries with extra
+                        edges at probability 0.5, 8 labels)
 """
 from __future__ import annotations
 
@@ -20,6 +23,13 @@ N_LABELS = 8
 RANDOM_BATCH_SCALE = 14  # labeled R-MAT scale of the random-query batch
 SEEDS = {"tc": 20240601, "clique": 20240520, "labeled": 20240607, "queries": 7,
          "batch": 20240613}
+# Batched matching: 256 queries of 4..10 vertices as one batch.  On R-MAT
+# with 8 labels their embeddings grow ~28x per scale (1.6e9 at scale 10,
+# 4.5e10 at 11; one query alone has 4.9e11 at scale 16), so the batch runs at
+# the largest scale an exact enumeration finishes in bench time.
+BATCH_QUERIES = 256
+BATCH_SCALE = 11
+BATCH_ORACLE_SCALE = 10
 
 NAMED_SHAPES = {
     "triangle": (3, [(0, 1), (1, 2), (0, 2)]),
@@ -188,3 +198,10 @@ def dense_queries(g: G.HostCSR, labels: np.ndarray, count: int, seed: int,
         e = np.asarray(sorted(edges), dtype=np.uint32).reshape(-1, 2)
         out.append((size, e, labels[np.asarray(members)].astype(np.uint32)))
     return out
+
+
+def batch_queries(count: int = BATCH_QUERIES, seed: int = SEEDS["batch"]):
+    """The batched-matching workload: seeded connected queries of 4..10
+    vertices (uneven sizes), a random spanning tree plus every other pair with
+    probability 0.5, labels uniform in [0, 8)."""
+    return random_queries(count, seed, min_size=4, max_size=10, extra_edge_prob=0.5)

@@ -390,6 +390,25 @@ typedef struct gpm_query {
   const uint32_t* labels;   /* n labels                                      */
 } gpm_query;
 
+/*
+ * How the matcher tests a streamed window of 64 candidates for membership in
+ * the other backward neighbours' label slices (the role of the reference's
+ * binary_search / intersect variants, src/graph/search.cuh:6-87 and
+ * src/gpu_kernels/tuning_schedules.cuh:169-202, chosen there at compile time).
+ */
+enum gpm_match_setv {
+  GPM_MATCH_SETV_AUTO = 0,    /* autotune: MERGE when the plan streams slices
+                                 of expected length >= 256 entries, else
+                                 BINARY                                      */
+  GPM_MATCH_SETV_BINARY = 1,  /* every lane binary-searches its candidate     */
+  GPM_MATCH_SETV_MERGE = 2,   /* the wave walks the probed slice in 64-entry
+                                 coalesced chunks from a per-stream cursor and
+                                 each lane finds its candidate among the
+                                 chunk's values; probes of short streams (< 4
+                                 windows) or of slices much longer than the
+                                 stream are binary-searched instead          */
+};
+
 typedef struct gpm_tuning {
   int autotune;             /* 1: project autotuner picks per-query chunks;
                                with policy LEARNED and no root_model also the
@@ -407,7 +426,10 @@ typedef struct gpm_tuning {
                                candidate orders and sizes chunks and
                                heavy-root parts.  NULL: label statistics and
                                the plan trial instead                        */
-  uint32_t reserved[2];
+  int set_variant;          /* gpm_match_setv: membership probes of the
+                               matcher; AUTO lets the autotuner pick per
+                               query (BINARY without autotune)               */
+  uint32_t reserved;
 } gpm_tuning;
 
 /*

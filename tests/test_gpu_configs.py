@@ -20,6 +20,10 @@ tests/test_gpu_tc_clique.py::test_tc_er_workload.  Here:
             learned and fixed-chunk graph replay agree at scale 18 (the four
             small queries) and pyramid-6 at the largest scale that runs in
             test time.
+  config 4  256 seeded connected queries of 4-10 vertices (8 labels) as one
+            batch on labeled R-MAT: oracle parity per query at scale 10
+            under every order, probe variant and graph replay; plain,
+            learned and both probe variants agree at the bench's scale.
 """
 import json
 import os
@@ -152,3 +156,34 @@ def test_config3_named_queries_s18(gpm, workloads):
         p = dg.match_batch(pyr, policy=0)
         print("config 3 pyramid-6 s16", int(p[0]), flush=True)
         assert np.array_equal(dg.match_batch(pyr, policy=1), p)
+
+
+def test_config4_batch_oracle(gpm, orc, workloads):
+    """Config 4's workload (256 seeded connected 4-10-vertex queries, 8
+    labels, one batch) at the scale the oracle enumerates in seconds: every
+    per-query count equals the oracle's under the plain and learned orders,
+    both membership-probe variants and fixed-chunk graph replay."""
+    g, labels = workloads.labeled_graph(workloads.BATCH_ORACLE_SCALE)
+    qs = workloads.batch_queries()
+    want = orc.ceci_batch(g.offsets, g.adj, labels, qs)
+    print("config 4 oracle total", int(want.sum()), flush=True)
+    assert np.count_nonzero(want) > 200
+    with gpm.DeviceGraph(g, oriented=False, labels=labels, n_labels=8) as dg:
+        for kw in (dict(policy=0), dict(policy=1), dict(policy=1, set_variant=1),
+                   dict(policy=1, set_variant=2), dict(autotune=0, chunk=8, use_graph=1)):
+            got = dg.match_batch(qs, **kw)
+            print("config 4", kw, dg.last_kernel_ms(), "ms", flush=True)
+            assert np.array_equal(got, want), kw
+
+
+def test_config4_batch_bench_scale(gpm, workloads):
+    """The same batch at the bench's scale: plain, learned and the forced
+    probe variants give identical per-query counts."""
+    g, labels = workloads.labeled_graph(workloads.BATCH_SCALE)
+    qs = workloads.batch_queries()
+    with gpm.DeviceGraph(g, oriented=False, labels=labels, n_labels=8) as dg:
+        learned = dg.match_batch(qs, policy=1)
+        print("config 4 scale", workloads.BATCH_SCALE, "total", int(learned.sum()),
+              dg.last_kernel_ms(), "ms", flush=True)
+        for kw in (dict(policy=0), dict(policy=1, set_variant=1)):
+            assert np.array_equal(dg.match_batch(qs, **kw), learned), kw

@@ -202,3 +202,40 @@ def test_root_work_and_features(gpm):
     hop = np.bincount(src, weights=deg[g.adj.astype(np.int64)], minlength=g.n)
     assert np.array_equal(f[cand, 2], hop[cand].astype(np.float32))
     assert np.all(w[~cand] == 0) and np.all(w[cand] > 0)
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+def test_set_variants_exact(gpm, hprd, orc, variant):
+    """Both membership-probe variants (BINARY, MERGE) and the autotuner's pick
+    give the reference's HPRD answers and the oracle's counts on graphs whose
+    slices span many 64-entry chunks: a dense seeded random graph with two
+    labels (probed slices of ~70 entries, misses between hits), a one-label
+    R-MAT graph with hubs (probed slices far longer than the stream: the
+    binary fallback), and K_120 (every probe hits; closed forms)."""
+    dg, qs, expected = hprd
+    assert np.array_equal(dg.match_batch(qs, set_variant=variant), expected)
+    assert np.array_equal(dg.match_batch(qs, autotune=0, chunk=4, set_variant=variant), expected)
+    z3 = np.zeros(3, dtype=np.uint32)
+    z4 = np.zeros(4, dtype=np.uint32)
+    tri = (3, np.array([[0, 1], [1, 2], [0, 2]], dtype=np.uint32), z3)
+    c4 = (4, np.array([[0, 1], [1, 2], [2, 3], [3, 0]], dtype=np.uint32), z4)
+    dia = (4, np.array([[0, 1], [0, 2], [0, 3], [1, 2], [1, 3]], dtype=np.uint32), z4)
+    k4 = (4, np.array([[0, 1], [0, 2], [0, 3], [1, 2], [1, 3], [2, 3]], dtype=np.uint32), z4)
+    rng = np.random.default_rng(21)
+    two = [(n, e, rng.integers(0, 2, n).astype(np.uint32)) for n, e, _ in (tri, c4, dia, k4)]
+    cases = ((graphs.er(300, 22000, 5), 2, two), (gpm.rmat_graph(11, 8, 77), 1, [tri, c4, dia, k4]),
+             (graphs.complete(120), 1, [tri, k4]))
+    for g, nl, qs2 in cases:
+        labels = rng.integers(0, nl, g.n).astype(np.uint32)
+        want = orc.ceci_batch(g.offsets, g.adj, labels, qs2)
+        with gpm.DeviceGraph(g, oriented=False, labels=labels, n_labels=nl) as d2:
+            assert np.array_equal(d2.match_batch(qs2, set_variant=variant), want)
+            assert np.array_equal(d2.match_batch(qs2, policy=1, set_variant=variant), want)
+            assert np.array_equal(d2.match_batch(qs2), want)  # the autotuner's pick
+    assert want.tolist() == [120 * 119 * 118, 120 * 119 * 118 * 117]
+
+
+def test_set_variant_rejected(gpm, hprd):
+    dg, qs, _ = hprd
+    with pytest.raises(gpm.GpmError):
+        dg.match_batch(qs[:1], set_variant=3)
