@@ -94,6 +94,7 @@ def roofline(dom, dom_ms, alg_bytes, traffic, lds):
         out.update({"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_GINSTR,
                     "unit": "G wave64 VALU instr/s", "frac": achieved / VALU_PEAK_GINSTR,
                     "valu_instr_per_launch": lds["valu_instr"],
+                    "valu_lane_utilization": lds.get("lane_util"),
                     "peak_basis": "1024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction",
                     "lds": {"bytes_per_launch": lds["bytes"], "achieved_GBs": lds_rate,
                             "peak_GBs": LDS_PEAK_GBS, "frac": lds_rate / LDS_PEAK_GBS,
@@ -281,8 +282,9 @@ def extra_sched(scale):
 def extra_patterns(scale_both, scale_counting):
     """P1-P8 (G2MinerSolver and PatternSolver): both formulations on R-MAT
     scale_both (counts must agree), the counting formulation and its BLOCK_OPT
-    prefilter on scale_counting, where the direct enumeration of P1-P3 runs
-    for minutes."""
+    prefilter on scale_counting.  The direct enumeration of P3 takes 18 s at
+    scale 12 and the counting formulation of P2 68 s at scale 16, so the rows
+    stay at scales 11 and 12."""
     out = {"workload": f"P1-P8 on R-MAT (edge factor 16, seed 20240522): both formulations at "
                        f"scale {scale_both}, counting formulation at scale {scale_counting}"}
     pids = ["P1", "P2", "P3", "P4", "P5", "P6", "P7", "P8"]
@@ -375,6 +377,19 @@ def extra_match(scale, big_scale):
         row["learned_speedup"] = row["plain"]["ms"] / row["learned"]["ms"]
         res["one_batch"] = row
     out["named"] = res
+    # The fifth named query, the 6-vertex pyramid, at scale 16: it has 3.1e9
+    # embeddings there and grows ~10x per two scales.
+    g, labels = W.labeled_graph(16)
+    res = {"workload": "pyramid-6 (wheel W6) on labeled R-MAT scale 16 (8 labels)"}
+    with G.DeviceGraph(g, oriented=False, labels=labels, n_labels=W.N_LABELS) as dg:
+        for pol, pname in ((G.SCHED_DEGREE, "plain"), (G.SCHED_LEARNED, "learned")):
+            t0 = time.perf_counter()
+            c = dg.match_batch([named["pyramid-6"]], autotune=1, policy=pol)
+            first = (time.perf_counter() - t0) * 1e3
+            res[pname] = {"count": int(c[0]), "ms": dg.last_kernel_ms(), "first_call_ms": first}
+        res["identical"] = res["plain"]["count"] == res["learned"]["count"]
+    out["pyramid6_s16"] = res
+    print("[bench] pyramid-6 done", file=sys.stderr, flush=True)
     # The same batch of four on the scale of the learned-scheduling workload.
     g, labels = W.labeled_graph(big_scale)
     res = {"workload": f"triangle, 4-cycle, diamond, three-star as one batch on labeled "
@@ -402,6 +417,32 @@ def extra_match(scale, big_scale):
         res["merge_vs_binary_speedup"] = res["plain_binary"]["ms"] / res["plain_merge"]["ms"]
     out["named_batch_large"] = res
     print("[bench] large named batch done", file=sys.stderr, flush=True)
+    # Batched matching workload: 256 seeded connected queries of 4-10
+    # vertices (8 labels) as one persistent launch; the autotuner picks each
+    # query's chunk and probe variant.  Scale: the largest at which the
+    # exact enumeration finishes in bench time (see workloads.BATCH_SCALE).
+    bs = W.BATCH_SCALE
+    qs = W.batch_queries()
+    g, labels = W.labeled_graph(bs)
+    res = {"workload": f"{len(qs)} seeded connected queries (4-10 vertices, 8 labels) on "
+                       f"labeled R-MAT scale {bs}, one batch",
+           "sizes": [int(q[0]) for q in qs]}
+    with G.DeviceGraph(g, oriented=False, labels=labels, n_labels=W.N_LABELS) as dg:
+        counts = {}
+        for pname, pol in (("learned", G.SCHED_LEARNED), ("plain", G.SCHED_DEGREE)):
+            t0 = time.perf_counter()
+            c = dg.match_batch(qs, autotune=1, policy=pol)
+            first = (time.perf_counter() - t0) * 1e3
+            ms = dg.last_kernel_ms()
+            counts[pname] = c
+            res[pname] = {"ms": ms, "first_call_ms": first,
+                          "embeddings_per_s": float(c.sum()) / (ms * 1e-3)}
+            print(f"[bench] batch of {len(qs)} {pname} {ms:.1f} ms", file=sys.stderr, flush=True)
+        res["identical"] = bool(np.array_equal(counts["plain"], counts["learned"]))
+        res["counts"] = [int(x) for x in counts["learned"]]
+        res["embeddings"] = int(counts["learned"].sum())
+        res["learned_speedup"] = res["plain"]["ms"] / res["learned"]["ms"]
+    out["batch_256"] = res
     # Uneven batch: seeded random connected queries (random spanning tree plus
     # each other pair with probability 0.5), 3-6 vertices, 8 labels.
     rs = W.RANDOM_BATCH_SCALE
@@ -569,7 +610,7 @@ def main():
         elif e == "match":
             out["extra"]["match"] = extra_match(args.match_scale, args.sched_scale)
         elif e == "patterns":
-            out["extra"]["patterns"] = extra_patterns(12, 16)
+            out["extra"]["patterns"] = extra_patterns(11, 12)
         print(f"[bench] extra {e} done in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
               flush=True)
     print(json.dumps(out), flush=True)

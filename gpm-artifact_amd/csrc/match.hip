@@ -894,9 +894,9 @@ bool make_plan(const gpm_graph* g, const HostQuery& q, int policy, QPlan* p) {
 
 constexpr size_t kMaxCandidates = 6;  // plans a trial compares per query
 
-// Distinct plans a trial compares for one query: the learned plan, the BFS
-// plan, and the cost-based orders from the other start vertices in order of
-// estimated work, at most kMaxCandidates.
+// Distinct plans a trial compares for one query: the BFS plan (the plain
+// policy's, kept on near-ties), then the cost-based orders from every start
+// vertex in order of estimated work, at most kMaxCandidates.
 std::vector<QPlan> plan_candidates(const gpm_graph* g, const HostQuery& q) {
   std::vector<QPlan> out;
   if (!labels_known(g, q)) return out;
@@ -916,9 +916,8 @@ std::vector<QPlan> plan_candidates(const gpm_graph* g, const HostQuery& q) {
   std::stable_sort(starts.begin(), starts.end(),
                    [](const auto& x, const auto& y) { return x.first < y.first; });
   if (starts.empty()) return out;
-  add(starts[0].second);
   add(bfs_order(g, q));
-  for (size_t i = 1; i < starts.size(); ++i) add(starts[i].second);
+  for (size_t i = 0; i < starts.size(); ++i) add(starts[i].second);
   return out;
 }
 
@@ -1395,40 +1394,42 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
     }
   }
   // Short batches (the autotuner, no model): when the tasks so far cannot
-  // keep kTasksPerWave tasks on every resident wave, the plans whose DFS goes
-  // below depth 2 are dealt out whole -- every root as tasks of a few
-  // depth-1 entries, each depth-2 stream of a task cut into nparts2 ranges
-  // -- so a few dense subtrees (small graphs, dense queries) spread over the
-  // chip.  About kRefineTasksPerWave tasks per resident wave in all.
+  // keep kTasksPerWave tasks on every resident wave, every plan with a DFS
+  // (tail >= 2) is dealt out whole -- every root as tasks of a few depth-1
+  // entries and, where the DFS goes below depth 2, each depth-2 stream of a
+  // task cut into nparts2 ranges -- so a few dense subtrees (small graphs,
+  // dense queries) spread over the chip.  About kRefineTasksPerWave tasks per
+  // resident wave in all.
   constexpr uint64_t kRefineTasksPerWave = 4;
   const uint64_t have_tasks = (uint64_t)t + heavy.size();
   bool refined = false;
   if (tun.autotune && !learned && !trial && !log_roots && !tun.chunk &&
       have_tasks < kTasksPerWave * waves) {
-    uint64_t e1 = 0;  // depth-1 entries of the roots of the refined plans
+    uint64_t e1 = 0, e1_deep = 0;  // depth-1 entries of the refined roots
     for (uint32_t li = 0; li < nq; ++li)
-      if (laid[li].tail > 2)
-        for (uint32_t r = 0; r < laid[li].n_roots; ++r) e1 += rlen[laid[li].root_off + r];
+      if (laid[li].tail >= 2)
+        for (uint32_t r = 0; r < laid[li].n_roots; ++r) {
+          e1 += rlen[laid[li].root_off + r];
+          if (laid[li].tail > 2) e1_deep += rlen[laid[li].root_off + r];
+        }
     const uint64_t target_tasks = kRefineTasksPerWave * waves;
     if (e1 > 0) {
       refined = true;
       const uint32_t part1 = (uint32_t)std::max<uint64_t>(1, (e1 + target_tasks - 1) / target_tasks);
-      const uint32_t nparts2 =
-          part1 > 1 ? 1u : (uint32_t)std::min<uint64_t>(16, (target_tasks + e1 - 1) / e1);
+      // Depth-2 ranges multiply the deep plans' tasks up to the target.
+      const uint32_t deep_parts =
+          part1 > 1 || e1_deep == 0
+              ? 1u
+              : (uint32_t)std::min<uint64_t>(
+                    16, std::max<uint64_t>(1, (target_tasks - (e1 - e1_deep) + e1_deep - 1) /
+                                                  e1_deep));
       marks.assign(std::max<uint64_t>(root_total, 1), 0u);
-      std::vector<HeavyTask> kept;
-      std::vector<double> kept_len;
-      for (size_t i = 0; i < heavy.size(); ++i)
-        if (laid[heavy[i].plan].tail <= 2) {  // parts of plans not refined stay
-          kept.push_back(heavy[i]);
-          kept_len.push_back(hlen[i]);
-          marks[laid[heavy[i].plan].root_off + heavy[i].root] = 0xFFFFFFFFu;
-        }
-      heavy.swap(kept);
-      hlen.swap(kept_len);
+      heavy.clear();
+      hlen.clear();
       for (uint32_t li = 0; li < nq; ++li) {
         const QPlan& q = laid[li];
-        if (q.tail <= 2) continue;
+        if (q.tail < 2) continue;
+        const uint32_t nparts2 = q.tail > 2 ? deep_parts : 1u;
         for (uint32_t r = 0; r < q.n_roots; ++r) {
           const uint32_t len = rlen[q.root_off + r];
           marks[q.root_off + r] = 0xFFFFFFFFu;  // no embeddings when len == 0
@@ -1604,32 +1605,22 @@ int plan_queries(gpm_graph* g, const gpm_query* queries, int n_queries, const gp
                          hipStreamSynchronize(g->stream) != hipSuccess))
       st = fail(GPM_EHIP, "plan trial");
   }
-  // Estimated time of a plan in dependent-load rounds: its total work spread
-  // over the resident waves, or its largest task, whichever is longer.
-  // Plans with fewer (estimated) tasks stopped at the budget rank first --
-  // their other figures are truncated; otherwise a later candidate replaces
-  // the current pick only when its estimate is lower by more than the
-  // sampling noise (kTrialMargin), so near-ties keep the earlier, better
-  // estimated plan.
-  const double waves = (double)t->grid * kWavesPerBlock;
-  free_batch(t);
-  if (st == GPM_EINVAL || st == GPM_ENOMEM) {
-    // The trial holds every candidate's roots at once; when that does not
-    // fit, the cost-based plans are kept untried.
-    clear_error();
-    return GPM_OK;
-  }
-  if (st != GPM_OK) return st;
+  // Estimated cost of a plan: its total weighted steps (sampled x scale).
+  // Heavy roots are dealt out in parts and short batches are cut further at
+  // depth 2, so a plan's largest sampled task is not its critical path.  A
+  // task stopped at the budget makes its plan's figure a lower bound, which
+  // still rules that plan out when it exceeds another plan's figure; ranking
+  // plans with fewer stopped tasks first instead picked, on pyramid-6 at
+  // R-MAT scale 16, a plan 2.7x slower than the BFS plan.  A later candidate
+  // replaces the current pick only when its estimate is lower by more than
+  // the sampling noise (kTrialMargin), so near-ties keep the earlier plan:
+  // the BFS plan, then the cost-based ones.
   for (const Tried& tr : tried) {
     size_t best = tr.first;
-    double best_est = 1e300;
-    double best_stopped = 1e300;
-    for (size_t i = tr.first; i < tr.first + tr.count; ++i) {
-      const double stopped = (double)work[3 * i + 2] * scale[i];
-      const double est =
-          std::max((double)work[3 * i] * scale[i] / waves, (double)work[3 * i + 1]);
-      if (stopped < best_stopped || (stopped == best_stopped && est < kTrialMargin * best_est)) {
-        best_stopped = stopped;
+    double best_est = (double)work[3 * tr.first] * scale[tr.first];
+    for (size_t i = tr.first + 1; i < tr.first + tr.count; ++i) {
+      const double est = (double)work[3 * i] * scale[i];
+      if (est < kTrialMargin * best_est) {
         best_est = est;
         best = i;
       }

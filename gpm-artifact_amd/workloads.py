@@ -25,11 +25,11 @@ SEEDS = {"tc": 20240601, "clique": 20240520, "labeled": 20240607, "queries": 7,
          "batch": 20240613}
 # Batched matching: 256 queries of 4..10 vertices as one batch.  On R-MAT
 # with 8 labels their embeddings grow ~28x per scale (1.6e9 at scale 10,
-# 4.5e10 at 11; one query alone has 4.9e11 at scale 16), so the batch runs at
-# the largest scale an exact enumeration finishes in bench time.
+# 4.5e10 at 11, where the batch takes ~45 s; one query alone has 4.9e11 at
+# scale 16), so the batch runs at scale 10, where the oracle also enumerates
+# it (in ~25 s on 8 threads).
 BATCH_QUERIES = 256
-BATCH_SCALE = 11
-BATCH_ORACLE_SCALE = 10
+BATCH_SCALE = 10
 
 NAMED_SHAPES = {
     "triangle": (3, [(0, 1), (1, 2), (0, 2)]),

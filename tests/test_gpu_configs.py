@@ -21,9 +21,9 @@ tests/test_gpu_tc_clique.py::test_tc_er_workload.  Here:
             small queries) and pyramid-6 at the largest scale that runs in
             test time.
   config 4  256 seeded connected queries of 4-10 vertices (8 labels) as one
-            batch on labeled R-MAT: oracle parity per query at scale 10
-            under every order, probe variant and graph replay; plain,
-            learned and both probe variants agree at the bench's scale.
+            batch on labeled R-MAT at the bench's scale (10): oracle parity
+            per query under the plain and learned orders, both probe
+            variants and fixed-chunk graph replay.
 """
 import json
 import os
@@ -160,10 +160,11 @@ def test_config3_named_queries_s18(gpm, workloads):
 
 def test_config4_batch_oracle(gpm, orc, workloads):
     """Config 4's workload (256 seeded connected 4-10-vertex queries, 8
-    labels, one batch) at the scale the oracle enumerates in seconds: every
-    per-query count equals the oracle's under the plain and learned orders,
-    both membership-probe variants and fixed-chunk graph replay."""
-    g, labels = workloads.labeled_graph(workloads.BATCH_ORACLE_SCALE)
+    labels, one batch) at the bench's scale, which the oracle enumerates in
+    seconds: every per-query count equals the oracle's under the plain and
+    learned orders, both membership-probe variants and fixed-chunk graph
+    replay."""
+    g, labels = workloads.labeled_graph(workloads.BATCH_SCALE)
     qs = workloads.batch_queries()
     want = orc.ceci_batch(g.offsets, g.adj, labels, qs)
     print("config 4 oracle total", int(want.sum()), flush=True)
@@ -175,15 +176,3 @@ def test_config4_batch_oracle(gpm, orc, workloads):
             print("config 4", kw, dg.last_kernel_ms(), "ms", flush=True)
             assert np.array_equal(got, want), kw
 
-
-def test_config4_batch_bench_scale(gpm, workloads):
-    """The same batch at the bench's scale: plain, learned and the forced
-    probe variants give identical per-query counts."""
-    g, labels = workloads.labeled_graph(workloads.BATCH_SCALE)
-    qs = workloads.batch_queries()
-    with gpm.DeviceGraph(g, oriented=False, labels=labels, n_labels=8) as dg:
-        learned = dg.match_batch(qs, policy=1)
-        print("config 4 scale", workloads.BATCH_SCALE, "total", int(learned.sum()),
-              dg.last_kernel_ms(), "ms", flush=True)
-        for kw in (dict(policy=0), dict(policy=1, set_variant=1)):
-            assert np.array_equal(dg.match_batch(qs, **kw), learned), kw
