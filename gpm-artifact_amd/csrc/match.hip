@@ -1393,18 +1393,22 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
       }
     }
   }
-  // Short batches (the autotuner, no model): when the tasks so far cannot
-  // keep kTasksPerWave tasks on every resident wave, every plan with a DFS
-  // (tail >= 2) is dealt out whole -- every root as tasks of a few depth-1
-  // entries and, where the DFS goes below depth 2, each depth-2 stream of a
-  // task cut into nparts2 ranges -- so a few dense subtrees (small graphs,
-  // dense queries) spread over the chip.  About kRefineTasksPerWave tasks per
-  // resident wave in all.
+  // Short batches (the autotuner, no model): when the tasks so far are
+  // fewer than kRefineBelow per resident wave, every plan with a DFS (tail
+  // >= 2) is dealt out whole -- every root as tasks of at most kSplitLen
+  // depth-1 entries and, where the DFS goes below depth 2, each depth-2
+  // stream of a task cut into nparts2 ranges -- so a few dense subtrees
+  // (small graphs, dense queries) spread over the chip.  About
+  // kRefineTasksPerWave tasks per resident wave in all, more where every
+  // root has more than kSplitLen depth-1 entries.  (With 2 tasks per wave as
+  // the bar, pyramid-6 on R-MAT scale 16 ran refined under one plan, 2.7 s,
+  // and unrefined under another, 7.2 s.)
   constexpr uint64_t kRefineTasksPerWave = 4;
+  constexpr uint64_t kRefineBelow = 8;
   const uint64_t have_tasks = (uint64_t)t + heavy.size();
   bool refined = false;
   if (tun.autotune && !learned && !trial && !log_roots && !tun.chunk &&
-      have_tasks < kTasksPerWave * waves) {
+      have_tasks < kRefineBelow * waves) {
     uint64_t e1 = 0, e1_deep = 0;  // depth-1 entries of the refined roots
     for (uint32_t li = 0; li < nq; ++li)
       if (laid[li].tail >= 2)
@@ -1415,7 +1419,8 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
     const uint64_t target_tasks = kRefineTasksPerWave * waves;
     if (e1 > 0) {
       refined = true;
-      const uint32_t part1 = (uint32_t)std::max<uint64_t>(1, (e1 + target_tasks - 1) / target_tasks);
+      const uint32_t part1 = (uint32_t)std::min<uint64_t>(
+          kSplitLen, std::max<uint64_t>(1, (e1 + target_tasks - 1) / target_tasks));
       // Depth-2 ranges multiply the deep plans' tasks up to the target.
       const uint32_t deep_parts =
           part1 > 1 || e1_deep == 0
