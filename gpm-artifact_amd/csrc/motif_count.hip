@@ -42,6 +42,10 @@ using namespace gpm;
 namespace {
 
 constexpr int kBlock = 256;
+// 4-cycle corners: threads per block and blocks per CU (the best of the
+// shapes measured at the launch).
+constexpr int kCycBlock = 256;
+constexpr int kCycBlocksPerCu = 1;
 
 // Blocks of 256 threads for `threads` threads, at most 32 per CU (the
 // kernels stride over the rest).
@@ -153,6 +157,7 @@ __device__ __forceinline__ uint32_t count_below(const uint32_t* __restrict__ a, 
 // N(u) and of every such N(v).  Atomic increments whose previous values sum
 // to sum_x C(cnt_x, 2); every x seen first is appended to `touched`
 // (*s_nt entries).
+template <int BS>
 __device__ __forceinline__ void cycle_walk(const uint32_t* __restrict__ off,
                                            const uint32_t* __restrict__ adj, uint32_t u,
                                            uint32_t* __restrict__ cnt,
@@ -161,7 +166,7 @@ __device__ __forceinline__ void cycle_walk(const uint32_t* __restrict__ off,
                                            uint32_t* s_wsum) {
   const uint32_t b = uniform(off[u]);
   const uint32_t dl = uniform(count_below(adj + b, uniform(off[u + 1]) - b, u));
-  for (uint32_t c0 = 0; c0 < dl; c0 += kBlock) {
+  for (uint32_t c0 = 0; c0 < dl; c0 += BS) {
     const uint32_t i = c0 + threadIdx.x;
     uint32_t len = 0, vb = 0;
     if (i < dl) {
@@ -179,9 +184,9 @@ __device__ __forceinline__ void cycle_walk(const uint32_t* __restrict__ off,
     s_pre[threadIdx.x] = incl0 + add;
     s_beg[threadIdx.x] = vb;
     __syncthreads();
-    const uint32_t total = uniform(s_pre[kBlock - 1]);
-    for (uint32_t f = threadIdx.x; f < total; f += kBlock) {
-      const uint32_t j = upper_index(s_pre, kBlock, f);
+    const uint32_t total = uniform(s_pre[BS - 1]);
+    for (uint32_t f = threadIdx.x; f < total; f += BS) {
+      const uint32_t j = upper_index(s_pre, BS, f);
       const uint32_t excl = j ? s_pre[j - 1] : 0u;
       const uint32_t x = adj[s_beg[j] + (f - excl)];
       const uint32_t old = atomicAdd(&cnt[x], 1u);
@@ -195,13 +200,14 @@ __device__ __forceinline__ void cycle_walk(const uint32_t* __restrict__ off,
 // Block per root, highest rank first (dynamic queue).  Each block owns
 // 2 n words at counters + blockIdx.x * stride: n counters, then the list of
 // the counters the current root touched.
-__global__ __launch_bounds__(kBlock) void cycles_kernel(
+template <int BS>
+__global__ __launch_bounds__(BS) void cycles_kernel(
     const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj, uint32_t n,
     unsigned int* __restrict__ queue, uint32_t* __restrict__ counters, uint64_t stride,
     unsigned long long* __restrict__ out) {
-  __shared__ uint32_t s_pre[kBlock];
-  __shared__ uint32_t s_beg[kBlock];
-  __shared__ uint32_t s_wsum[kBlock / kWave];
+  __shared__ uint32_t s_pre[BS];
+  __shared__ uint32_t s_beg[BS];
+  __shared__ uint32_t s_wsum[BS / kWave];
   __shared__ uint32_t s_root;
   __shared__ uint32_t s_nt;
   uint32_t* cnt = counters + (uint64_t)blockIdx.x * stride;
@@ -215,13 +221,13 @@ __global__ __launch_bounds__(kBlock) void cycles_kernel(
     __syncthreads();
     if (q >= n) break;
     const uint32_t u = n - 1 - q;
-    cycle_walk(off, adj, u, cnt, touched, &s_nt, &acc, s_pre, s_beg, s_wsum);
+    cycle_walk<BS>(off, adj, u, cnt, touched, &s_nt, &acc, s_pre, s_beg, s_wsum);
     // Counter updates and list appends of this root must be complete before
     // the listed counters are cleared; the fence orders them.
     __threadfence();
     __syncthreads();
     const uint32_t nt = uniform(s_nt);
-    for (uint32_t i = threadIdx.x; i < nt; i += kBlock) cnt[touched[i]] = 0u;
+    for (uint32_t i = threadIdx.x; i < nt; i += BS) cnt[touched[i]] = 0u;
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) s_nt = 0u;
@@ -332,13 +338,15 @@ int motif4_by_identities(gpm_graph* g, uint64_t out[6]) {
   if ((st = ensure_src(g->dag)) != GPM_OK) return st;
   const int cus = device_cus(g);
   // Scratch: per block, n 4-cycle counters and a list of up to n touched
-  // counters; two blocks per CU, within 8 GiB (fewer blocks when that much
-  // is not free).  The wedge increments are random atomics into these
-  // arrays: more blocks keep more of them in flight but spread them over
-  // more memory -- on R-MAT scale 22, 8 blocks per CU (64 GiB of counters)
-  // took 2.97 s against 1.96 s with 2.
+  // counters; kCycBlocksPerCu blocks of kCycBlock threads per CU, within
+  // 8 GiB (fewer blocks when that much is not free).  The wedge increments
+  // are random atomics into these arrays: more or wider blocks keep more of
+  // them in flight but spread them over more memory.  4-cycle corners on
+  // R-MAT scale 22 (blocks x threads): 256 x 256 1.96 s, 128 x 256 2.62 s,
+  // 512 x 256 2.24 s, 1024 x 256 2.91 s, 2048 x 256 2.97 s, 256 x 512 2.47 s,
+  // 256 x 1024 3.64 s, 128 x 1024 3.44 s.
   const uint64_t stride = 2 * (uint64_t)std::max<uint32_t>(g->n, 1);
-  unsigned blocks_c = (unsigned)std::min<uint64_t>((uint64_t)cus * 2, std::max(g->n, 1u));
+  unsigned blocks_c = (unsigned)std::min<uint64_t>((uint64_t)cus * kCycBlocksPerCu, std::max(g->n, 1u));
   while (blocks_c > 16 && (uint64_t)blocks_c * stride * 4 > (8ull << 30)) blocks_c /= 2;
   // The per-edge triangle counts (one uint32 per DAG edge) use the same
   // scratch before the 4-cycle counters do.
@@ -374,8 +382,8 @@ int motif4_by_identities(gpm_graph* g, uint64_t out[6]) {
   const uint64_t zero_n = cyc_bytes / 4;
   hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_of(g, zero_n)), dim3(256), 0, g->stream, ccnt,
                      zero_n);
-  hipLaunchKernelGGL(cycles_kernel, dim3(blocks_c), dim3(kBlock), 0, g->stream, g->rk_off,
-                     g->rk_adj, g->n, queue, ccnt, stride, g->counters + 5);
+  hipLaunchKernelGGL(cycles_kernel<kCycBlock>, dim3(blocks_c), dim3(kCycBlock), 0, g->stream,
+                     g->rk_off, g->rk_adj, g->n, queue, ccnt, stride, g->counters + 5);
   GPM_HIP(hipGetLastError());
   timing_mark(g, "motif_cycles");
   if ((st = timing_end(g)) != GPM_OK) return st;
