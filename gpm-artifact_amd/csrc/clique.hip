@@ -302,8 +302,10 @@ __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, 
     for (uint32_t p = threadIdx.x; p < total; p += BS) {
       const uint32_t jj = upper_index(s_pre, BS, p);
       uint32_t kth = p - (jj ? s_pre[jj - 1] : 0u);
-      const unsigned long long* ri = rows + (size_t)(part + (c0 + jj) * n_parts) * RS;
       if (MAXW > 0) {
+        // LDS rows: row index < 1024 and stride < 32, so the 24-bit multiply
+        // (full rate; a 32-bit multiply issues at quarter rate) is exact.
+        const unsigned long long* ri = rows + __umul24(part + (c0 + jj) * n_parts, RS);
         uint64_t wi[MAXW > 0 ? MAXW : 1];
 #pragma unroll
         for (int x = 0; x < MAXW; ++x) wi[x] = x < (int)W ? ri[x] : 0ull;
@@ -320,7 +322,7 @@ __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, 
             kth -= pw;
           }
         }
-        const unsigned long long* rj = rows + (size_t)j * RS;
+        const unsigned long long* rj = rows + __umul24(j, RS);
         uint64_t m[MAXW > 0 ? MAXW : 1];
 #pragma unroll
         for (int x = 0; x < MAXW; ++x) m[x] = wi[x] & rj[x];
@@ -343,22 +345,26 @@ __device__ __forceinline__ uint64_t count_pairs(const unsigned long long* rows, 
               // non-zero (fewer LDS reads; 33.1 -> 29.5 ms per fused pass on
               // R-MAT s20); the 16-word tier measured faster with every
               // word read under one wait.
-              const unsigned long long* rl = rows + (size_t)l * RS;
+              // One 32-bit sum per l (at most 16 x 64 bits), one 64-bit add.
+              const unsigned long long* rl = rows + __umul24(l, RS);
+              uint32_t cl = 0;
               if (MAXW <= 8) {
 #pragma unroll
                 for (int y = x; y < MAXW; ++y)
-                  if (m[y]) c += __popcll(m[y] & rl[y]);
+                  if (m[y]) cl += (uint32_t)__popcll(m[y] & rl[y]);
               } else {
                 uint64_t a[MAXW > 0 ? MAXW : 1];
 #pragma unroll
                 for (int y = x; y < MAXW; ++y) a[y] = rl[y];
 #pragma unroll
-                for (int y = x; y < MAXW; ++y) c += __popcll(m[y] & a[y]);
+                for (int y = x; y < MAXW; ++y) cl += (uint32_t)__popcll(m[y] & a[y]);
               }
+              c += cl;
             }
           }
         }
       } else {
+        const unsigned long long* ri = rows + (size_t)(part + (c0 + jj) * n_parts) * RS;
         uint32_t w = 0;
         while (true) {
           const uint32_t pw = (uint32_t)__popcll(ri[w]);
@@ -543,7 +549,8 @@ __global__ __launch_bounds__(BS) void clique_block_kernel(
             const uint32_t p = upper_index(S, d, x);
             idx = (p > 0 && S[p - 1] == x) ? (int)(p - 1) : -1;
           }
-          if (idx >= 0) atomicOr(&rows[(size_t)row_i * RS + (idx >> 6)], 1ull << (idx & 63));
+          // Rows are this block's alone (LDS, or its slice of HBM scratch).
+          if (idx >= 0) wg_atomic_or(&rows[(size_t)row_i * RS + (idx >> 6)], 1ull << (idx & 63));
         }
       }
       __syncthreads();
@@ -565,7 +572,7 @@ __global__ __launch_bounds__(BS) void clique_block_kernel(
           const uint32_t k = f / d, t = f - k * d;
           const uint32_t ri = s_pre[k];
           if (sorted_contains(adj + begs[ri], lens[ri], S[t]))
-            atomicOr(&rows[(size_t)ri * RS + (t >> 6)], 1ull << (t & 63));
+            wg_atomic_or(&rows[(size_t)ri * RS + (t >> 6)], 1ull << (t & 63));
         }
         __syncthreads();
       }

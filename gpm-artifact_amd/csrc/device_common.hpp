@@ -23,6 +23,19 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Atomics on global memory that only the calling workgroup reads or writes
+// (block-private scratch).  Workgroup scope lets the L2 perform them; at the
+// default agent scope this multi-XCD part sends every one to memory (the
+// counters show every L2 atomic forwarded to the memory side), which made the
+// Motif4 4-cycle counters' random increments the cost of the kernel.  Pair
+// them with __threadfence_block(), not __threadfence().
+__device__ __forceinline__ uint32_t wg_atomic_add(uint32_t* p, uint32_t v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void wg_atomic_or(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Number of set bits of `mask` strictly below this lane (v_mbcnt_lo/hi).
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),

@@ -189,7 +189,8 @@ __device__ __forceinline__ void cycle_walk(const uint32_t* __restrict__ off,
     // kWedgesInFlight wedges per thread per step: their endpoint loads and
     // counter atomics are independent, so their latencies overlap (the walk
     // runs few waves per CU: one block per CU holds the counters).  On R-MAT
-    // scale 22: 1 in flight 1.96 s, 4 1.85 s, 8 1.88 s.  Aggregating a
+    // scale 22 (agent-scope atomics): 1 in flight 1.96 s, 4 1.85 s, 8 1.88 s;
+    // with workgroup-scope atomics 1.39 s.  Aggregating a
     // root's increments in an LDS hash table first (4.5 wedges per distinct
     // endpoint on the heaviest roots) measured 3.01 s: the table's inserts
     // and the barriers of its flushes cost more than the atomics saved.
@@ -207,7 +208,7 @@ __device__ __forceinline__ void cycle_walk(const uint32_t* __restrict__ off,
       }
 #pragma unroll
       for (uint32_t k = 0; k < kWedgesInFlight; ++k)
-        old[k] = x[k] != kEmpty ? atomicAdd(&cnt[x[k]], 1u) : 1u;
+        old[k] = x[k] != kEmpty ? wg_atomic_add(&cnt[x[k]], 1u) : 1u;
 #pragma unroll
       for (uint32_t k = 0; k < kWedgesInFlight; ++k) {
         if (x[k] == kEmpty) continue;
@@ -245,12 +246,14 @@ __global__ __launch_bounds__(BS) void cycles_kernel(
     const uint32_t u = n - 1 - q;
     cycle_walk<BS>(off, adj, u, cnt, touched, &s_nt, &acc, s_pre, s_beg, s_wsum);
     // Counter updates and list appends of this root must be complete before
-    // the listed counters are cleared; the fence orders them.
-    __threadfence();
+    // the listed counters are cleared, and the clears before the next root's
+    // increments; the counters are this block's alone, so workgroup-scope
+    // fences order them.
+    __threadfence_block();
     __syncthreads();
     const uint32_t nt = uniform(s_nt);
     for (uint32_t i = threadIdx.x; i < nt; i += BS) cnt[touched[i]] = 0u;
-    __threadfence();
+    __threadfence_block();
     __syncthreads();
     if (threadIdx.x == 0) s_nt = 0u;
   }
