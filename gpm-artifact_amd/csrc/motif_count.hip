@@ -189,8 +189,8 @@ __device__ __forceinline__ void cycle_walk(const uint32_t* __restrict__ off,
     // kWedgesInFlight wedges per thread per step: their endpoint loads and
     // counter atomics are independent, so their latencies overlap (the walk
     // runs few waves per CU: one block per CU holds the counters).  On R-MAT
-    // scale 22 (agent-scope atomics): 1 in flight 1.96 s, 4 1.85 s, 8 1.88 s;
-    // with workgroup-scope atomics 1.39 s.  Aggregating a
+    // scale 22 (device-scope fences per root): 1 in flight 1.96 s, 4 1.85 s,
+    // 8 1.88 s; with workgroup-scope atomics and fences 1.39 s.  Aggregating a
     // root's increments in an LDS hash table first (4.5 wedges per distinct
     // endpoint on the heaviest roots) measured 3.01 s: the table's inserts
     // and the barriers of its flushes cost more than the atomics saved.
