@@ -17,9 +17,9 @@ tests/test_gpu_tc_clique.py::test_tc_er_workload.  Here:
             the oracle's counts.
   config 3  the five named queries (triangle, 4-cycle, diamond, three-star,
             pyramid-6) on labeled R-MAT: oracle parity at scale 12; plain,
-            learned and fixed-chunk graph replay agree at scale 18 (the four
-            small queries) and pyramid-6 at the largest scale that runs in
-            test time.
+            learned and fixed-chunk graph replay agree at scale 18 for all
+            five (pyramid-6 there: 6.0e10 embeddings, about 70 s per run) and
+            at scale 16.
   config 4  256 seeded connected queries of 4-10 vertices (8 labels) as one
             batch on labeled R-MAT at the bench's scale (10): oracle parity
             per query under the plain and learned orders, both probe
@@ -156,6 +156,29 @@ def test_config3_named_queries_s18(gpm, workloads):
         p = dg.match_batch(pyr, policy=0)
         print("config 3 pyramid-6 s16", int(p[0]), flush=True)
         assert np.array_equal(dg.match_batch(pyr, policy=1), p)
+
+
+@pytest.fixture(scope="module")
+def pyramid_s18(gpm, workloads):
+    """pyramid-6 on labeled R-MAT scale 18 (config 3's nominal scale): the
+    graph handle and the plain count (6.0e10 embeddings, ~70 s)."""
+    g, labels = workloads.labeled_graph(18)
+    pyr = [dict(workloads.named_queries())["pyramid-6"]]
+    dg = gpm.DeviceGraph(g, oriented=False, labels=labels, n_labels=8)
+    plain = dg.match_batch(pyr, policy=0)
+    print("config 3 pyramid-6 s18", int(plain[0]), dg.last_kernel_ms(), "ms", flush=True)
+    yield dg, pyr, plain
+    dg.close()
+
+
+def test_config3_pyramid_s18_learned(pyramid_s18):
+    dg, pyr, plain = pyramid_s18
+    assert np.array_equal(dg.match_batch(pyr, policy=1), plain)
+
+
+def test_config3_pyramid_s18_graph_replay(pyramid_s18):
+    dg, pyr, plain = pyramid_s18
+    assert np.array_equal(dg.match_batch(pyr, autotune=0, chunk=8, use_graph=1), plain)
 
 
 def test_config4_batch_oracle(gpm, orc, workloads):
