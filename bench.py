@@ -314,7 +314,7 @@ def extra_patterns(scale_both, scale_counting):
     return out
 
 
-def extra_match(scale, big_scale):
+def extra_match(scale, big_scale, with_cpu=False):
     """Labeled matching: the reference's 200 HPRD queries as one batch (exact
     against the reference's known answers), and the named queries on a
     labeled R-MAT graph."""
@@ -442,6 +442,18 @@ def extra_match(scale, big_scale):
         res["counts"] = [int(x) for x in counts["learned"]]
         res["embeddings"] = int(counts["learned"].sum())
         res["learned_speedup"] = res["plain"]["ms"] / res["learned"]["ms"]
+    if with_cpu:
+        # The oracle's CECI restatement over the same batch, parallel across
+        # the queries (each query sequential, as in the reference).
+        from oracle import oracle as O
+        t0 = time.perf_counter()
+        want = O.ceci_batch(g.offsets, g.adj, labels, qs)
+        t = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": float(want.sum()) / t, "unit": "embeddings/s",
+                               "seconds": t, "cores": O.num_threads(), "kind": "port",
+                               "parallel": "across queries",
+                               "exact_vs_engine": bool(np.array_equal(want, counts["learned"]))}
+        print(f"[bench] batch of {len(qs)} cpu oracle {t:.1f} s", file=sys.stderr, flush=True)
     out["batch_256"] = res
     # Uneven batch: seeded random connected queries (random spanning tree plus
     # each other pair with probability 0.5), 3-6 vertices, 8 labels.
@@ -608,7 +620,8 @@ def main():
         elif e == "sched":
             out["extra"]["sched"] = extra_sched(args.sched_scale)
         elif e == "match":
-            out["extra"]["match"] = extra_match(args.match_scale, args.sched_scale)
+            out["extra"]["match"] = extra_match(args.match_scale, args.sched_scale,
+                                                not args.no_cpu)
         elif e == "patterns":
             out["extra"]["patterns"] = extra_patterns(11, 12)
         print(f"[bench] extra {e} done in {time.perf_counter() - t0:.1f}s", file=sys.stderr,

@@ -30,6 +30,24 @@ def test_tc_er_workload(gpm, orc):
             assert dg.tc_repeat(gpm.TC_EDGE, reps) == want
 
 
+@pytest.mark.parametrize("per_vertex", [5, 10, 20, 40, 80, 160])
+def test_edge_group_sizes(gpm, orc, per_vertex):
+    """The edge kernel's group size G follows m / n (4, 8, 16, 32, 64, 64
+    here); lists of up to 2G entries take the lane-exchange path, longer ones
+    the binary search, and an Erdos-Renyi graph mixes both in every wave.
+    Single counts, hipGraph repeats and fused repeats (4 edges per group)
+    against the oracle."""
+    n = 3000
+    d = gpm.orient(gpm.er_graph(n, n * per_vertex, 500 + per_vertex))
+    want = orc.tc(d.offsets, d.adj)
+    assert want > 0
+    with gpm.DeviceGraph(d, oriented=True) as dg:
+        assert dg.tc(gpm.TC_EDGE) == want
+        assert dg.tc_repeat(gpm.TC_EDGE, 33) == want
+        assert dg.tc_repeat_fused(33) == want
+        assert dg.tc_repeat_fused(7) == want
+
+
 @pytest.mark.parametrize("n", [3, 4, 40, 65, 300, 700])
 def test_complete_graphs(gpm, n):
     # K_n exercises every tier: out-degrees reach n-1 (wave <= 64, LDS block
