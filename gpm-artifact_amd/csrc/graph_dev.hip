@@ -224,6 +224,7 @@ void release(gpm_graph* g) {
   (void)hipFree(g->bmask);
   hipFree(g->scratch);
   hipFree(g->counters);
+  (void)hipFree(g->tc_partial);
   for (auto& e : g->mark_ev)
     if (e) (void)hipEventDestroy(e);
   if (g->ev0) hipEventDestroy(g->ev0);

@@ -68,9 +68,11 @@ struct gpm_graph {
   gpm_graph* dag = nullptr;
   // Last prepared matching batch (match.hip), replayed on identical calls.
   void* match_cache = nullptr;
-  // Instantiated hipGraph of repeated edge-parallel triangle counts (tc.hip).
+  // Instantiated hipGraph of repeated edge-parallel triangle counts, and the
+  // per-block partial sums of the edge kernel (tc.hip).
   hipGraphExec_t tc_exec = nullptr;
   hipGraph_t tc_graph = nullptr;
+  unsigned long long* tc_partial = nullptr;
   // ORIENTED graphs: the same DAG relabelled in a topological order
   // (toff/tadj, every edge from a lower to a higher id; h_rank[v] = new id
   // of v), built on first use by the k-clique kernels.

@@ -26,13 +26,15 @@ using namespace gpm;
 
 // Scratch of gpm_tc_edge_noalloc: device memory the library never
 // allocates -- each GPU holds its own copy of this array from the moment the
-// code object is loaded until the process ends.  Layout: one uint64 counter
-// at byte 0, then the COO source ids (uint32) of up to kNoAllocMaxEdges
-// oriented edges from byte kNoAllocSrcOffset.  Host code must not use the
-// array's name as a device address: it resolves the per-device address with
+// code object is loaded until the process ends.  Layout: the edge kernel's
+// per-block partial sums (uint64, up to kNoAllocMaxBlocks) from byte 0, then
+// the COO source ids (uint32) of up to kNoAllocMaxEdges oriented edges from
+// byte kNoAllocSrcOffset.  Host code must not use the array's name as a
+// device address: it resolves the per-device address with
 // hipGetSymbolAddress once per device (noalloc_scratch below).
 constexpr uint64_t kNoAllocMaxEdges = 16ull << 20;  // 16,777,216 oriented edges
-constexpr size_t kNoAllocSrcOffset = 256;
+constexpr uint64_t kNoAllocMaxBlocks = 8192;        // edge-kernel blocks (partial sums)
+constexpr size_t kNoAllocSrcOffset = kNoAllocMaxBlocks * sizeof(unsigned long long);
 constexpr size_t kNoAllocScratchBytes = kNoAllocSrcOffset + kNoAllocMaxEdges * sizeof(uint32_t);
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
@@ -43,6 +45,9 @@ constexpr uint32_t kTcWaveCap = 256;     // roots served by one wave
 constexpr uint32_t kTcWaveTable = 512;   // LDS keys per wave
 constexpr uint32_t kTcBlockTableMax = 8192;  // 32 KiB of keys per block
 constexpr uint64_t kTcFusedEdgesPerGroup = 4;  // edges per group, fused repeat
+constexpr uint64_t kTcEdgeBlocksPerCu = 32;    // edge-kernel grid cap per CU
+constexpr int kTcGraphReps = 32;   // counts per launch of the repeat graph
+constexpr int kTcGraphChains = 8;  // independent chains the counts form
 
 // v from the lane whose group index is (this lane's) ^ X, for X < 32: a
 // ds_swizzle in bit-mask mode (and 0x1F, or 0, xor X) needs no lane-address
@@ -74,10 +79,19 @@ __device__ __forceinline__ bool group_has(uint32_t a, uint32_t b) {
   }
 }
 
+// Edge-parallel count; block (x, y) stores the sum of its lanes' counts in
+// partial[y * gridDim.x + x] (a plain store, every block, so the array needs
+// no clearing), and tc_sum_kernel adds them up.  A launch with gridDim.y = R
+// runs R independent counts (the fused repeat of gpm_tc_repeat_fused).
+// One atomic per triangle-holding wave into one counter measured slower:
+// atomics to a single address serialise (about 8 ns each here), which made
+// the count on Erdos-Renyi n = 16384, M = 131072 (666 triangles) grow with
+// its triangles rather than its edges.
 template <int G>
 __global__ __launch_bounds__(kBlock) void tc_edge_kernel(
     const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
-    const uint32_t* __restrict__ src, uint64_t m, unsigned long long* out) {
+    const uint32_t* __restrict__ src, uint64_t m, unsigned long long* __restrict__ partial) {
+  __shared__ unsigned long long s_wave[kBlock / kWave];
   const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t sub = threadIdx.x & (G - 1);
   const uint64_t ngrp = (uint64_t)gridDim.x * blockDim.x / G;
@@ -116,12 +130,34 @@ __global__ __launch_bounds__(kBlock) void tc_edge_kernel(
     }
   }
   // Sparse graphs leave most waves without a triangle: those skip the
-  // cross-lane sum and the atomic.
-  if (__ballot(cnt != 0) == 0) return;
-  cnt = wave_sum_u64(cnt);
-  // A launch with gridDim.y = R runs R independent counts, count y into
-  // out[y] (the fused repeat of gpm_tc_repeat); gridDim.y = 1 is one count.
-  if (lane_id() == 0) atomicAdd(out + blockIdx.y, (unsigned long long)cnt);
+  // cross-lane sum.
+  if (__ballot(cnt != 0) != 0) cnt = wave_sum_u64(cnt);
+  if (lane_id() == 0) s_wave[threadIdx.x / kWave] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long b = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) b += s_wave[w];
+    partial[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = b;
+  }
+}
+
+// out[y] = sum of partial[y * nb, (y + 1) * nb): one block per count, a plain
+// store (the counter needs no clearing).
+__global__ __launch_bounds__(kBlock) void tc_sum_kernel(
+    const unsigned long long* __restrict__ partial, uint32_t nb,
+    unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long s_wave[kBlock / kWave];
+  const unsigned long long* p = partial + (size_t)blockIdx.x * nb;
+  uint64_t v = 0;
+  for (uint32_t i = threadIdx.x; i < nb; i += kBlock) v += p[i];
+  v = wave_sum_u64(v);
+  if (lane_id() == 0) s_wave[threadIdx.x / kWave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) t += s_wave[w];
+    out[blockIdx.x] = t;
+  }
 }
 
 // Wave per root: roots[] lists vertices with 2 <= |N+| <= kTcWaveCap.
@@ -274,20 +310,9 @@ __global__ void noalloc_src_kernel(const uint32_t* __restrict__ off, uint32_t n,
   }
 }
 
-// *dst = *src (one thread): hands the scratch counter to the caller's output.
-__global__ void copy_u64_kernel(const unsigned long long* __restrict__ src,
-                                unsigned long long* __restrict__ dst) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = *src;
-}
-
 __global__ void zero_u32_kernel(unsigned int* p, int count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) p[i] = 0u;
-}
-
-__global__ void zero_u64_kernel(unsigned long long* p, int count) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) p[i] = 0ull;
 }
 
 int edge_group_size(uint32_t n, uint64_t m) {
@@ -301,10 +326,41 @@ int edge_group_size(uint32_t n, uint64_t m) {
 
 // Grid of the edge kernel: one G-lane group per edges_per_group oriented
 // edges, at most 32 blocks per CU (the groups then stride over the edges).
-uint64_t edge_blocks(uint64_t m, int G, int cus, uint64_t edges_per_group = 1) {
+uint64_t edge_blocks(uint64_t m, int G, int cus, uint64_t edges_per_group, uint64_t max_blocks) {
   const uint64_t groups = (std::max<uint64_t>(m, 1) + edges_per_group - 1) / edges_per_group;
-  return std::max<uint64_t>(
-      1, std::min<uint64_t>((groups * G + kBlock - 1) / kBlock, (uint64_t)cus * 32));
+  const uint64_t cap = std::min<uint64_t>((uint64_t)cus * kTcEdgeBlocksPerCu, max_blocks);
+  return std::max<uint64_t>(1, std::min<uint64_t>((groups * G + kBlock - 1) / kBlock, cap));
+}
+
+const void* edge_kernel(int G) {
+  switch (G) {
+    case 4: return reinterpret_cast<const void*>(&tc_edge_kernel<4>);
+    case 8: return reinterpret_cast<const void*>(&tc_edge_kernel<8>);
+    case 16: return reinterpret_cast<const void*>(&tc_edge_kernel<16>);
+    case 32: return reinterpret_cast<const void*>(&tc_edge_kernel<32>);
+    default: return reinterpret_cast<const void*>(&tc_edge_kernel<64>);
+  }
+}
+
+hipError_t launch_edge(int G, dim3 grid, hipStream_t stream, const uint32_t* off,
+                       const uint32_t* adj, const uint32_t* src, uint64_t m,
+                       unsigned long long* partial) {
+  void* args[] = {&off, &adj, &src, &m, &partial};
+  return hipLaunchKernel(edge_kernel(G), grid, dim3(kBlock), args, 0, stream);
+}
+
+// Per-block partial sums of the edge kernel: kTcGraphReps counts of up to
+// kTcEdgeBlocksPerCu blocks per CU each, kept for the handle's lifetime (the
+// repeat graph's nodes hold the address).
+int ensure_tc_partial(gpm_graph* g) {
+  if (g->tc_partial) return GPM_OK;
+  const size_t n = (size_t)device_cus(g) * kTcEdgeBlocksPerCu * kTcGraphReps;
+  if (hipMalloc(&g->tc_partial, n * sizeof(unsigned long long)) != hipSuccess) {
+    g->tc_partial = nullptr;
+    (void)hipGetLastError();
+    return fail(GPM_ENOMEM, "triangle-count partial sums (%zu entries)", n);
+  }
+  return GPM_OK;
 }
 
 // Enqueues one triangle count (counter reset + kernels) on g's stream; the
@@ -313,18 +369,16 @@ int enqueue_tc(gpm_graph* g, int variant, bool mark) {
   int st;
   const int cus = device_cus(g);
   if (variant == GPM_TC_EDGE) {
-    if ((st = zero_counters(g, 1)) != GPM_OK) return st;
+    if ((st = ensure_tc_partial(g)) != GPM_OK) return st;
     const int G = edge_group_size(g->n, g->m);
-    dim3 grid((unsigned)edge_blocks(g->m, G, cus)), block(kBlock);
-    switch (G) {
-      case 4: hipLaunchKernelGGL(tc_edge_kernel<4>, grid, block, 0, g->stream, g->off, g->adj, g->src, g->m, g->counters); break;
-      case 8: hipLaunchKernelGGL(tc_edge_kernel<8>, grid, block, 0, g->stream, g->off, g->adj, g->src, g->m, g->counters); break;
-      case 16: hipLaunchKernelGGL(tc_edge_kernel<16>, grid, block, 0, g->stream, g->off, g->adj, g->src, g->m, g->counters); break;
-      case 32: hipLaunchKernelGGL(tc_edge_kernel<32>, grid, block, 0, g->stream, g->off, g->adj, g->src, g->m, g->counters); break;
-      default: hipLaunchKernelGGL(tc_edge_kernel<64>, grid, block, 0, g->stream, g->off, g->adj, g->src, g->m, g->counters); break;
-    }
-    GPM_HIP(hipGetLastError());
+    const uint64_t blocks = edge_blocks(g->m, G, cus, 1, ~0ull);
+    GPM_HIP(launch_edge(G, dim3((unsigned)blocks), g->stream, g->off, g->adj, g->src, g->m,
+                        g->tc_partial));
     if (mark) timing_mark(g, "tc_edge");
+    hipLaunchKernelGGL(tc_sum_kernel, dim3(1), dim3(kBlock), 0, g->stream, g->tc_partial,
+                       (uint32_t)blocks, g->counters);
+    GPM_HIP(hipGetLastError());
+    if (mark) timing_mark(g, "tc_sum");
     return GPM_OK;
   }
   const RootTiers* t = nullptr;
@@ -357,52 +411,44 @@ int enqueue_tc(gpm_graph* g, int variant, bool mark) {
   return GPM_OK;
 }
 
-constexpr int kTcGraphReps = 32;   // counts per launch of the repeat graph
-constexpr int kTcGraphChains = 8;  // independent chains the counts form
-
-// hipGraph of kTcGraphReps edge-parallel counts, each into its own counter
-// slot, after one node that clears the slots: count r depends only on count
-// r - kTcGraphChains, so the graph is kTcGraphChains independent chains the
-// device may run side by side.  A launch measures the counting throughput of
-// short launches without per-launch host cost (8 chains of 4: 8.4 us per
-// count on Erdos-Renyi n = 16384, M = 131072; 32 unordered nodes: 8.7 us).
+// hipGraph of kTcGraphReps edge-parallel counts, count r as its edge kernel
+// (partial sums in slice r of g->tc_partial) then its sum into counter slot
+// r.  Count r starts after count r - kTcGraphChains, so the graph is
+// kTcGraphChains independent chains the device may run side by side.  A
+// launch measures the counting throughput of short launches without
+// per-launch host cost.
 int ensure_tc_graph(gpm_graph* g) {
   if (g->tc_exec) return GPM_OK;
-  GPM_HIP(hipGraphCreate(&g->tc_graph, 0));
-  unsigned long long* counters = g->counters;
-  int nslots = kTcGraphReps;
-  void* zargs[] = {&counters, &nslots};
-  hipKernelNodeParams zp{};
-  zp.func = reinterpret_cast<void*>(&zero_u64_kernel);
-  zp.gridDim = dim3(1);
-  zp.blockDim = dim3(64);
-  zp.kernelParams = zargs;
-  hipGraphNode_t zn = nullptr;
-  GPM_HIP(hipGraphAddKernelNode(&zn, g->tc_graph, nullptr, 0, &zp));
+  int st;
+  if ((st = ensure_tc_partial(g)) != GPM_OK) return st;
   const int G = edge_group_size(g->n, g->m);
   const int cus = device_cus(g);
-  const uint64_t blocks = edge_blocks(g->m, G, cus);
-  void* func = nullptr;
-  switch (G) {
-    case 4: func = reinterpret_cast<void*>(&tc_edge_kernel<4>); break;
-    case 8: func = reinterpret_cast<void*>(&tc_edge_kernel<8>); break;
-    case 16: func = reinterpret_cast<void*>(&tc_edge_kernel<16>); break;
-    case 32: func = reinterpret_cast<void*>(&tc_edge_kernel<32>); break;
-    default: func = reinterpret_cast<void*>(&tc_edge_kernel<64>); break;
-  }
+  const uint64_t blocks = edge_blocks(g->m, G, cus, 1, ~0ull);
+  const size_t slice = (size_t)cus * kTcEdgeBlocksPerCu;
+  GPM_HIP(hipGraphCreate(&g->tc_graph, 0));
   const uint32_t *off = g->off, *adj = g->adj, *src = g->src;
   uint64_t m = g->m;
-  hipGraphNode_t nodes[kTcGraphReps] = {};
+  uint32_t nb = (uint32_t)blocks;
+  hipGraphNode_t sums[kTcGraphReps] = {};
   for (int r = 0; r < kTcGraphReps; ++r) {
+    unsigned long long* part = g->tc_partial + r * slice;
     unsigned long long* slot = g->counters + r;
-    void* args[] = {&off, &adj, &src, &m, &slot};
-    hipKernelNodeParams kp{};
-    kp.func = func;
-    kp.gridDim = dim3((unsigned)blocks);
-    kp.blockDim = dim3(kBlock);
-    kp.kernelParams = args;
-    const hipGraphNode_t dep = r < kTcGraphChains ? zn : nodes[r - kTcGraphChains];
-    GPM_HIP(hipGraphAddKernelNode(&nodes[r], g->tc_graph, &dep, 1, &kp));
+    void* eargs[] = {&off, &adj, &src, &m, &part};
+    hipKernelNodeParams ep{};
+    ep.func = const_cast<void*>(edge_kernel(G));
+    ep.gridDim = dim3((unsigned)blocks);
+    ep.blockDim = dim3(kBlock);
+    ep.kernelParams = eargs;
+    hipGraphNode_t en = nullptr;
+    const hipGraphNode_t* dep = r < kTcGraphChains ? nullptr : &sums[r - kTcGraphChains];
+    GPM_HIP(hipGraphAddKernelNode(&en, g->tc_graph, dep, dep ? 1 : 0, &ep));
+    void* sargs[] = {&part, &nb, &slot};
+    hipKernelNodeParams sp{};
+    sp.func = reinterpret_cast<void*>(&tc_sum_kernel);
+    sp.gridDim = dim3(1);
+    sp.blockDim = dim3(kBlock);
+    sp.kernelParams = sargs;
+    GPM_HIP(hipGraphAddKernelNode(&sums[r], g->tc_graph, &en, 1, &sp));
   }
   GPM_HIP(hipGraphInstantiate(&g->tc_exec, g->tc_graph, nullptr, nullptr, 0));
   return GPM_OK;
@@ -417,29 +463,25 @@ int tc_fused_repeat(gpm_graph* g, int reps, uint64_t* out) {
   GPM_CHECK_ARG(reps >= 1, "reps must be >= 1 (got %d)", reps);
   int st;
   if ((st = ensure_src(g)) != GPM_OK) return st;
+  if ((st = ensure_tc_partial(g)) != GPM_OK) return st;
   const int G = edge_group_size(g->n, g->m);
   const int cus = device_cus(g);
   // The counts of a launch fill the device together, so each group takes
   // kTcFusedEdgesPerGroup edges: fewer waves, each amortising its set-up
-  // and final sum over more edges (6.4 against 7.0 us per count on
-  // Erdos-Renyi n = 16384, M = 131072, 32 counts per launch).
-  const uint64_t blocks = edge_blocks(g->m, G, cus, kTcFusedEdgesPerGroup);
+  // and final sum over more edges.
+  const uint64_t blocks = edge_blocks(g->m, G, cus, kTcFusedEdgesPerGroup, ~0ull);
   unsigned long long h[kTcGraphReps] = {};
   if ((st = timing_begin(g)) != GPM_OK) return st;
   int done = 0;
   while (done < reps) {
     const int batch = std::min(kTcGraphReps, reps - done);
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, g->stream, g->counters, batch);
-    dim3 grid((unsigned)blocks, (unsigned)batch), block(kBlock);
-    switch (G) {
-      case 4: hipLaunchKernelGGL(tc_edge_kernel<4>, grid, block, 0, g->stream, g->off, g->adj, g->src, g->m, g->counters); break;
-      case 8: hipLaunchKernelGGL(tc_edge_kernel<8>, grid, block, 0, g->stream, g->off, g->adj, g->src, g->m, g->counters); break;
-      case 16: hipLaunchKernelGGL(tc_edge_kernel<16>, grid, block, 0, g->stream, g->off, g->adj, g->src, g->m, g->counters); break;
-      case 32: hipLaunchKernelGGL(tc_edge_kernel<32>, grid, block, 0, g->stream, g->off, g->adj, g->src, g->m, g->counters); break;
-      default: hipLaunchKernelGGL(tc_edge_kernel<64>, grid, block, 0, g->stream, g->off, g->adj, g->src, g->m, g->counters); break;
-    }
-    GPM_HIP(hipGetLastError());
+    // batch * blocks partial sums: within the kTcGraphReps slices.
+    GPM_HIP(launch_edge(G, dim3((unsigned)blocks, (unsigned)batch), g->stream, g->off, g->adj,
+                        g->src, g->m, g->tc_partial));
     if (done == 0) timing_mark(g, "tc_edge_fused");
+    hipLaunchKernelGGL(tc_sum_kernel, dim3(batch), dim3(kBlock), 0, g->stream, g->tc_partial,
+                       (uint32_t)blocks, g->counters);
+    GPM_HIP(hipGetLastError());
     done += batch;
     if (done >= reps) {
       GPM_HIP(hipMemcpyAsync(h, g->counters, batch * sizeof(h[0]), hipMemcpyDeviceToHost,
@@ -527,15 +569,16 @@ int tc_edge_noalloc(const uint32_t* off, const uint32_t* adj, uint32_t n, uint64
   void* base = nullptr;
   int st;
   if ((st = noalloc_scratch(&base)) != GPM_OK) return st;
-  unsigned long long* counter = static_cast<unsigned long long*>(base);
+  unsigned long long* partial = static_cast<unsigned long long*>(base);
   uint32_t* src = reinterpret_cast<uint32_t*>(static_cast<unsigned char*>(base) + kNoAllocSrcOffset);
   int dev = 0, cus = 256;
   GPM_HIP(hipGetDevice(&dev));
   GPM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  // The counter keeps the previous call's value: clear it with a kernel (no
-  // memset node, so the sequence may be captured into a hipGraph).
-  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, stream, counter, 1);
-  GPM_HIP(hipGetLastError());
+  // Every launched block stores its partial sum and the sum kernel stores
+  // the count, so nothing keeps a previous call's value (and no memset node
+  // is needed: the sequence may be captured into a hipGraph).  With m = 0
+  // one block of the edge kernel runs no edge and stores 0.
+  uint64_t blocks = 1;
   if (m) {
     const uint64_t sblocks = std::max<uint64_t>(
         1, std::min<uint64_t>(((uint64_t)n * kWave + kBlock - 1) / kBlock, (uint64_t)cus * 16));
@@ -543,18 +586,13 @@ int tc_edge_noalloc(const uint32_t* off, const uint32_t* adj, uint32_t n, uint64
                        n, m, src);
     GPM_HIP(hipGetLastError());
     const int G = edge_group_size(n, m);
-    const uint64_t blocks = edge_blocks(m, G, cus);
-    dim3 grid((unsigned)blocks), block(kBlock);
-    switch (G) {
-      case 4: hipLaunchKernelGGL(tc_edge_kernel<4>, grid, block, 0, stream, off, adj, src, m, counter); break;
-      case 8: hipLaunchKernelGGL(tc_edge_kernel<8>, grid, block, 0, stream, off, adj, src, m, counter); break;
-      case 16: hipLaunchKernelGGL(tc_edge_kernel<16>, grid, block, 0, stream, off, adj, src, m, counter); break;
-      case 32: hipLaunchKernelGGL(tc_edge_kernel<32>, grid, block, 0, stream, off, adj, src, m, counter); break;
-      default: hipLaunchKernelGGL(tc_edge_kernel<64>, grid, block, 0, stream, off, adj, src, m, counter); break;
-    }
-    GPM_HIP(hipGetLastError());
+    blocks = edge_blocks(m, G, cus, 1, kNoAllocMaxBlocks);
+    GPM_HIP(launch_edge(G, dim3((unsigned)blocks), stream, off, adj, src, m, partial));
+  } else {
+    GPM_HIP(launch_edge(4, dim3(1), stream, off, adj, src, 0, partial));
   }
-  hipLaunchKernelGGL(copy_u64_kernel, dim3(1), dim3(64), 0, stream, counter, count);
+  hipLaunchKernelGGL(tc_sum_kernel, dim3(1), dim3(kBlock), 0, stream, partial, (uint32_t)blocks,
+                     count);
   GPM_HIP(hipGetLastError());
   return GPM_OK;
 }

@@ -274,11 +274,12 @@ int gpm_tc_repeat_fused(gpm_graph* g, int reps, uint64_t* out);
  * the device uint64 *d_count.  The call only enqueues kernels on `stream` (a
  * hipStream_t; NULL = the null stream) and returns: *d_count holds the count
  * once the stream has reached that point.  Its staging (the COO source of
- * every edge and the counter) lives in a fixed device array of
- * 256 + 4 * 16,777,216 bytes that each GPU keeps until the process ends;
- * m > 16,777,216 is rejected with GPM_EINVAL.  Every call on one GPU shares
- * that array: calls that overlap on one GPU (two host threads, or two
- * streams running at once) race and must be serialised by the caller.
+ * every edge and the kernel's per-block partial sums) lives in a fixed
+ * device array of 8 * 8,192 + 4 * 16,777,216 bytes that each GPU keeps until
+ * the process ends; m > 16,777,216 is rejected with GPM_EINVAL.  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (two host
+ * threads, or two streams running at once) race and must be serialised by
+ * the caller.
  */
 int gpm_tc_edge_noalloc(const uint32_t* d_offsets, const uint32_t* d_adj, uint32_t n,
                         uint64_t m, uint64_t* d_count, void* stream);
