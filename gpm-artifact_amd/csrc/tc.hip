@@ -141,13 +141,13 @@ __global__ __launch_bounds__(kBlock) void tc_edge_kernel(
   }
 }
 
-// out[y] = sum of partial[y * nb, (y + 1) * nb): one block per count, a plain
-// store (the counter needs no clearing).
+// out[y] = sum of partial[y * stride, y * stride + nb): one block per count,
+// a plain store (the counter needs no clearing).
 __global__ __launch_bounds__(kBlock) void tc_sum_kernel(
-    const unsigned long long* __restrict__ partial, uint32_t nb,
+    const unsigned long long* __restrict__ partial, uint32_t nb, uint32_t stride,
     unsigned long long* __restrict__ out) {
   __shared__ unsigned long long s_wave[kBlock / kWave];
-  const unsigned long long* p = partial + (size_t)blockIdx.x * nb;
+  const unsigned long long* p = partial + (size_t)blockIdx.x * stride;
   uint64_t v = 0;
   for (uint32_t i = threadIdx.x; i < nb; i += kBlock) v += p[i];
   v = wave_sum_u64(v);
@@ -376,7 +376,7 @@ int enqueue_tc(gpm_graph* g, int variant, bool mark) {
                         g->tc_partial));
     if (mark) timing_mark(g, "tc_edge");
     hipLaunchKernelGGL(tc_sum_kernel, dim3(1), dim3(kBlock), 0, g->stream, g->tc_partial,
-                       (uint32_t)blocks, g->counters);
+                       (uint32_t)blocks, (uint32_t)blocks, g->counters);
     GPM_HIP(hipGetLastError());
     if (mark) timing_mark(g, "tc_sum");
     return GPM_OK;
@@ -411,12 +411,12 @@ int enqueue_tc(gpm_graph* g, int variant, bool mark) {
   return GPM_OK;
 }
 
-// hipGraph of kTcGraphReps edge-parallel counts, count r as its edge kernel
-// (partial sums in slice r of g->tc_partial) then its sum into counter slot
-// r.  Count r starts after count r - kTcGraphChains, so the graph is
-// kTcGraphChains independent chains the device may run side by side.  A
-// launch measures the counting throughput of short launches without
-// per-launch host cost.
+// hipGraph of kTcGraphReps edge-parallel counts: count r's edge kernel
+// stores its partial sums in slice r of g->tc_partial, and count r starts
+// after count r - kTcGraphChains, so the counts form kTcGraphChains
+// independent chains the device may run side by side.  One sum node after
+// the chains adds every slice into its counter slot.  A launch measures the
+// counting throughput of short launches without per-launch host cost.
 int ensure_tc_graph(gpm_graph* g) {
   if (g->tc_exec) return GPM_OK;
   int st;
@@ -428,28 +428,32 @@ int ensure_tc_graph(gpm_graph* g) {
   GPM_HIP(hipGraphCreate(&g->tc_graph, 0));
   const uint32_t *off = g->off, *adj = g->adj, *src = g->src;
   uint64_t m = g->m;
-  uint32_t nb = (uint32_t)blocks;
-  hipGraphNode_t sums[kTcGraphReps] = {};
+  hipGraphNode_t nodes[kTcGraphReps] = {};
   for (int r = 0; r < kTcGraphReps; ++r) {
     unsigned long long* part = g->tc_partial + r * slice;
-    unsigned long long* slot = g->counters + r;
     void* eargs[] = {&off, &adj, &src, &m, &part};
     hipKernelNodeParams ep{};
     ep.func = const_cast<void*>(edge_kernel(G));
     ep.gridDim = dim3((unsigned)blocks);
     ep.blockDim = dim3(kBlock);
     ep.kernelParams = eargs;
-    hipGraphNode_t en = nullptr;
-    const hipGraphNode_t* dep = r < kTcGraphChains ? nullptr : &sums[r - kTcGraphChains];
-    GPM_HIP(hipGraphAddKernelNode(&en, g->tc_graph, dep, dep ? 1 : 0, &ep));
-    void* sargs[] = {&part, &nb, &slot};
-    hipKernelNodeParams sp{};
-    sp.func = reinterpret_cast<void*>(&tc_sum_kernel);
-    sp.gridDim = dim3(1);
-    sp.blockDim = dim3(kBlock);
-    sp.kernelParams = sargs;
-    GPM_HIP(hipGraphAddKernelNode(&sums[r], g->tc_graph, &en, 1, &sp));
+    const hipGraphNode_t* dep = r < kTcGraphChains ? nullptr : &nodes[r - kTcGraphChains];
+    GPM_HIP(hipGraphAddKernelNode(&nodes[r], g->tc_graph, dep, dep ? 1 : 0, &ep));
   }
+  // The slices are `slice` apart: the sum kernel reads the first `blocks`
+  // entries of each (its stride argument is the slice, its length blocks).
+  unsigned long long* part = g->tc_partial;
+  unsigned long long* counters = g->counters;
+  uint32_t nb = (uint32_t)blocks, stride = (uint32_t)slice;
+  void* sargs[] = {&part, &nb, &stride, &counters};
+  hipKernelNodeParams sp{};
+  sp.func = reinterpret_cast<void*>(&tc_sum_kernel);
+  sp.gridDim = dim3(kTcGraphReps);
+  sp.blockDim = dim3(kBlock);
+  sp.kernelParams = sargs;
+  hipGraphNode_t sn = nullptr;
+  GPM_HIP(hipGraphAddKernelNode(&sn, g->tc_graph, &nodes[kTcGraphReps - kTcGraphChains],
+                                kTcGraphChains, &sp));
   GPM_HIP(hipGraphInstantiate(&g->tc_exec, g->tc_graph, nullptr, nullptr, 0));
   return GPM_OK;
 }
@@ -480,7 +484,7 @@ int tc_fused_repeat(gpm_graph* g, int reps, uint64_t* out) {
                         g->src, g->m, g->tc_partial));
     if (done == 0) timing_mark(g, "tc_edge_fused");
     hipLaunchKernelGGL(tc_sum_kernel, dim3(batch), dim3(kBlock), 0, g->stream, g->tc_partial,
-                       (uint32_t)blocks, g->counters);
+                       (uint32_t)blocks, (uint32_t)blocks, g->counters);
     GPM_HIP(hipGetLastError());
     done += batch;
     if (done >= reps) {
@@ -592,7 +596,7 @@ int tc_edge_noalloc(const uint32_t* off, const uint32_t* adj, uint32_t n, uint64
     GPM_HIP(launch_edge(4, dim3(1), stream, off, adj, src, 0, partial));
   }
   hipLaunchKernelGGL(tc_sum_kernel, dim3(1), dim3(kBlock), 0, stream, partial, (uint32_t)blocks,
-                     count);
+                     (uint32_t)blocks, count);
   GPM_HIP(hipGetLastError());
   return GPM_OK;
 }
