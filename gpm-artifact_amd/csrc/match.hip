@@ -186,6 +186,8 @@ struct WaveState {
   // MERGE probes: per probed slice, a position before which every entry is
   // below the rest of the anchor stream (advanced window by window).
   uint32_t pcur[kMaxQ][kMaxQ - 1];
+  // Range cut of the depth-d streams of the current task (make_cut).
+  uint32_t cut[kMaxQ];
 };
 
 // Sets up depth d: picks the backward neighbour with the shortest label slice.
@@ -417,8 +419,38 @@ constexpr uint64_t kTrialBudget = 1ull << 20;
 
 // Kernel modes: count embeddings; plan trial (weighted steps per plan,
 // budget-limited); root log (weighted steps of every task, one root per
-// task, no budget -- the training data of the learned root-cost model).
-constexpr int kModeCount = 0, kModeTrial = 1, kModeLog = 2;
+// task, no budget -- the training data of the learned root-cost model);
+// budgeted count (a task that reaches the pass budget of unweighted steps
+// stops, adds nothing and is listed, to be counted again cut finer).
+constexpr int kModeCount = 0, kModeTrial = 1, kModeLog = 2, kModeBudget = 3;
+
+// Step accounting: trial and log weigh a window or descent by its dependent
+// loads; counting modes add 1.
+__host__ __device__ constexpr bool weighted_steps(int mode) {
+  return mode == kModeTrial || mode == kModeLog;
+}
+
+// Budget of a task in a budgeted count: the task stops once its steps reach
+// `limit` while the launch's queue holds no task for an idle wave; while
+// the queue still has tasks, the limit is raised by `quantum` instead (so
+// tasks stop only in the launch's tail, when idle waves could take their
+// parts in the next launch).
+struct TaskBudget {
+  uint64_t limit, quantum;
+  const unsigned int* queue;
+  uint32_t n_tasks;
+};
+
+__device__ __forceinline__ bool over_budget(uint64_t steps, TaskBudget& b) {
+  if (steps < b.limit) return false;
+  const uint32_t q =
+      uniform(__hip_atomic_load(b.queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  if (q < b.n_tasks) {
+    b.limit = steps + b.quantum;
+    return false;
+  }
+  return true;
+}
 
 // Trial cost of one 64-entry window at depth d: the dependent loads one
 // lane makes -- the entry, its degree and NLF counts, and per probed
@@ -444,7 +476,8 @@ __device__ uint64_t window_cost(const DevGraph& g, const QPlan& p, const WaveSta
 // first member (each x tested against the later members' full filters).
 // Arithmetic is modulo 2^64, exact when the true count fits.
 template <int kMode, bool kMerge>
-__device__ uint64_t count_tail(const DevGraph& g, const QPlan& p, WaveState& s, uint64_t& steps) {
+__device__ uint64_t count_tail(const DevGraph& g, const QPlan& p, WaveState& s, uint64_t& steps,
+                               TaskBudget& budget) {
   const uint32_t lane = lane_id();
   uint64_t prod = 1;
   for (uint32_t gi = 0; gi < p.n_groups; ++gi) {
@@ -455,12 +488,13 @@ __device__ uint64_t count_tail(const DevGraph& g, const QPlan& p, WaveState& s, 
       init_depth(g, p, s, d);
       const uint32_t sl = uniform(s.sl[d]), sb = uniform(s.sb[d]);
       uint64_t wcost = 1;
-      if (kMode != kModeCount) {
+      if (weighted_steps(kMode)) {
         wcost = window_cost(g, p, s, d, false);
         for (uint32_t m2 = m + 1; m2 < k; ++m2) wcost += window_cost(g, p, s, p.grp_dep[gi][m2], true);
       }
       for (uint32_t cur = 0; cur < sl; cur += kWave) {
         if (kMode == kModeTrial && steps >= kTrialBudget) break;
+        if (kMode == kModeBudget && over_budget(steps, budget)) break;
         steps += wcost;
         const uint32_t i = cur + lane;
         const bool live = i < sl;
@@ -490,19 +524,19 @@ __device__ uint64_t count_tail(const DevGraph& g, const QPlan& p, WaveState& s, 
 
 // Counts the embeddings of plan p rooted at v0 whose depth-1 vertex is one
 // of the entries [lo1, hi1) of the depth-1 slice (all: 0, ~0u): a DFS over
-// depths 1..tail-1, each leaf of which adds the tail count.  With nparts2 >
-// 1 (plans whose DFS goes below depth 2 only) every depth-2 stream is cut
-// into nparts2 equal ranges of its anchor slice and only range part2 is
-// walked: the parts of one depth-1 entry partition its subtree.
+// depths 1..tail-1, each leaf of which adds the tail count.  s.cut[d] =
+// part << 16 | parts with parts > 1 cuts every depth-d stream of the DFS
+// into `parts` equal ranges of its anchor slice, of which only range `part`
+// is walked: the parts partition the subtree.
 template <int kMode, bool kMerge>
 __device__ uint64_t count_root(const DevGraph& g, const QPlan& p, WaveState& s, uint32_t v0,
-                               uint32_t lo1, uint32_t hi1, uint32_t part2, uint32_t nparts2,
-                               uint64_t& steps) {
+                               uint32_t lo1, uint32_t hi1, uint64_t& steps,
+                               TaskBudget& budget) {
   const uint32_t lane = lane_id();
   const uint32_t n = p.n;
   if (n == 1) return 1;
   s.emb[0] = v0;
-  if (p.tail == 1) return count_tail<kMode, kMerge>(g, p, s, steps);
+  if (p.tail == 1) return count_tail<kMode, kMerge>(g, p, s, steps, budget);
   uint64_t total = 0;
   init_depth(g, p, s, 1);
   s.cur[1] = lo1;
@@ -510,6 +544,7 @@ __device__ uint64_t count_root(const DevGraph& g, const QPlan& p, WaveState& s, 
   int d = 1;
   while (d >= 1) {
     if (kMode == kModeTrial && steps >= kTrialBudget) break;
+    if (kMode == kModeBudget && over_budget(steps, budget)) break;
     const uint32_t dd = (uint32_t)d;
     if (s.mask[dd] == 0ull) {
       const uint32_t cur = s.cur[dd];
@@ -517,7 +552,7 @@ __device__ uint64_t count_root(const DevGraph& g, const QPlan& p, WaveState& s, 
         --d;
         continue;
       }
-      steps += kMode != kModeCount ? window_cost(g, p, s, dd, false) : 1u;
+      steps += weighted_steps(kMode) ? window_cost(g, p, s, dd, false) : 1u;
       const uint32_t i = cur + lane;
       const bool live = i < s.sl[dd];
       const bool ok = window_passes<kMerge>(g, p, s, dd, live ? g.ladj[s.sb[dd] + i] : 0u, live);
@@ -531,15 +566,17 @@ __device__ uint64_t count_root(const DevGraph& g, const QPlan& p, WaveState& s, 
     s.mask[dd] = m & (m - 1);
     s.emb[dd] = uniform(g.ladj[s.sb[dd] + s.base[dd] + j]);
     if (dd + 1 == p.tail) {
-      total += count_tail<kMode, kMerge>(g, p, s, steps);
+      total += count_tail<kMode, kMerge>(g, p, s, steps, budget);
     } else {
       init_depth(g, p, s, dd + 1);
-      if (dd == 1 && nparts2 > 1) {
-        const uint64_t sl2 = s.sl[2];
-        s.cur[2] = (uint32_t)(sl2 * part2 / nparts2);
-        s.sl[2] = (uint32_t)(sl2 * (part2 + 1) / nparts2);
+      const uint32_t cut = uniform(s.cut[dd + 1]);
+      if ((cut & 0xFFFFu) > 1u) {
+        const uint64_t sl = s.sl[dd + 1];
+        const uint32_t part = cut >> 16, parts = cut & 0xFFFFu;
+        s.cur[dd + 1] = (uint32_t)(sl * part / parts);
+        s.sl[dd + 1] = (uint32_t)(sl * (part + 1) / parts);
       }
-      steps += kMode != kModeCount ? 2u * __builtin_popcount(p.bn[dd + 1]) : 1u;
+      steps += weighted_steps(kMode) ? 2u * __builtin_popcount(p.bn[dd + 1]) : 1u;
       ++d;
     }
   }
@@ -558,11 +595,35 @@ __device__ __forceinline__ uint32_t find_plan(const QPlan* plans, uint32_t nq, u
   return lo;
 }
 
+// Depths 2 .. kCutLevels + 1 of a task's DFS can be cut into ranges.
+constexpr int kCutLevels = 4;
+constexpr uint32_t kNoCut = 1u;  // part 0 of 1: the whole stream
+
+// Budgeted passes of tuned batches: steps (windows + descents) between a
+// running task's checks of the queue, parts per stopped task and level, and
+// the pass limit.  Passes run by default on short batches only (those the
+// refinement deals out whole): on long ones no task stops, and the budgeted
+// kernel, at 90 VGPRs (5 waves per SIMD against the count kernel's 6), took
+// 1.62 s where one count launch took 1.41 s (four named queries, labeled
+// R-MAT scale 22).  On dense 9-10-vertex queries of the config-4 generator a
+// quantum of 512 steps measured 51 ms, 4096 steps 105 ms.
+constexpr uint64_t kPassBudget = 512;
+constexpr uint32_t kPassBudgetOff = 0xFFFFFFFFu;  // gpm_tuning.pass_budget: one launch
+constexpr uint32_t kPassSplit = 16;
+constexpr int kMaxPasses = 8;
+constexpr size_t kMaxPassTasks = size_t(1) << 22;
+
+__host__ __device__ constexpr uint32_t make_cut(uint32_t part, uint32_t parts) {
+  return part << 16 | parts;
+}
+
 // A part of one heavy root: depth-1 entries [lo, hi) of root `root` (index
-// into the plan's root list) of plan `plan`.
+// into the plan's root list) of plan `plan`; cut[i] = make_cut(part, parts)
+// restricts every depth-(i + 2) stream of the part's DFS to range `part` of
+// `parts` (kNoCut: all of it).
 struct HeavyTask {
   uint32_t plan, root, lo, hi;
-  uint32_t part2, nparts2;  // depth-2 range of the task (0, 1: all)
+  uint32_t cut[kCutLevels];
 };
 
 // Persistent enumeration: every wave pulls tasks from one queue until it is
@@ -575,16 +636,21 @@ struct HeavyTask {
 // work[3 qid + 1] and the number of stopped tasks to work[3 qid + 2].
 // kMerge: the batch has MERGE plans (a launch without them is built without
 // the merge walk, which keeps its register count at 6 waves per SIMD).
+// kModeBudget: a task whose steps reach its budget (TaskBudget, quantum
+// `budget`) is cut short, adds no count and appends its index to
+// aborted[1 ..] (aborted[0] counts them; the list has room for every task).
 template <int kMode, bool kMerge>
 __global__ __launch_bounds__(kBlock) void match_kernel(
     DevGraph g, const QPlan* __restrict__ plans, uint32_t nq, uint32_t n_tasks,
     const uint32_t* __restrict__ roots, const uint32_t* __restrict__ rlen,
     const HeavyTask* __restrict__ heavy, uint32_t n_heavy, uint32_t split_len,
     unsigned int* __restrict__ queue, unsigned long long* __restrict__ counts,
-    unsigned long long* __restrict__ work) {
+    unsigned long long* __restrict__ work, uint64_t budget, unsigned int* __restrict__ aborted) {
   __shared__ WaveState s_state[kWavesPerBlock];
   WaveState& s = s_state[threadIdx.x / kWave];
   const uint32_t lane = lane_id();
+  // Every lane writes the same values (no lane-dependent branch).
+  for (int d = 0; d < kMaxQ; ++d) s.cut[d] = kNoCut;
   while (true) {
     uint32_t t = 0;
     if (lane == 0) t = atomicAdd(queue, 1u);
@@ -593,32 +659,55 @@ __global__ __launch_bounds__(kBlock) void match_kernel(
     uint64_t c = 0;
     uint32_t qid;
     uint64_t steps = 0;
+    TaskBudget tb{budget, budget, queue, n_tasks};
     if (t < n_heavy) {
       const HeavyTask h = heavy[t];
       const QPlan& p = plans[uniform(h.plan)];
       qid = p.qid;
+#pragma unroll
+      for (int i = 0; i < kCutLevels; ++i) s.cut[i + 2] = uniform(h.cut[i]);
       c = count_root<kMode, kMerge>(g, p, s, uniform(roots[p.root_off + uniform(h.root)]),
-                                    uniform(h.lo), uniform(h.hi), uniform(h.part2),
-                                    uniform(h.nparts2), steps);
+                                    uniform(h.lo), uniform(h.hi), steps, tb);
     } else {
       const uint32_t pi = uniform(find_plan(plans, nq, t - n_heavy));
       const QPlan& p = plans[pi];
       qid = p.qid;
       const uint32_t r0 = (t - n_heavy - p.task_off) * p.chunk;
       const uint32_t r1 = min(r0 + p.chunk, p.n_roots);
+#pragma unroll
+      for (int i = 0; i < kCutLevels; ++i) s.cut[i + 2] = kNoCut;
       for (uint32_t r = r0; r < r1; ++r) {
         if (p.tail > 1 && uniform(rlen[p.root_off + r]) > split_len) continue;  // a heavy task
         c += count_root<kMode, kMerge>(g, p, s, uniform(roots[p.root_off + r]), 0u, 0xFFFFFFFFu,
-                                       0u, 1u, steps);
+                                       steps, tb);
+        if (kMode == kModeBudget && over_budget(steps, tb)) break;
       }
     }
-    if (lane == 0 && c) atomicAdd(&counts[qid], (unsigned long long)c);
+    // A task that stopped early ends with steps >= its limit (the limit is
+    // only raised below the steps taken); one that merely crossed it on its
+    // last step is also counted again (exact, only some work repeated).
+    // Every lane holds the same step count; readfirstlane makes the test a
+    // scalar branch.
+    const bool stopped =
+        kMode == kModeBudget && __builtin_amdgcn_readfirstlane(steps >= tb.limit ? 1u : 0u) != 0u;
+    if (stopped) {
+      if (lane == 0) aborted[1 + atomicAdd(aborted, 1u)] = t;
+    } else if (lane == 0 && c) {
+      atomicAdd(&counts[qid], (unsigned long long)c);
+    }
     if (kMode == kModeLog && lane == 0) work[t] = steps;
     if (kMode == kModeTrial && lane == 0 && steps) {
       atomicAdd(&work[3 * qid], (unsigned long long)steps);
       atomicMax(&work[3 * qid + 1], (unsigned long long)steps);
       if (steps >= kTrialBudget) atomicAdd(&work[3 * qid + 2], 1ull);
     }
+    // Keeps the task loop one loop with one latch.  Without it the compiler
+    // gave some modes (root log, budgeted count) a second latch behind the
+    // lane-0 branches above, i.e. a nested loop around the task fetch's
+    // broadcast that lane 0 left alone; the other lanes then read task
+    // indices from the inactive lane 0 and never drained the queue (the root
+    // log's stall).  tests/test_match_isa.py checks every mode's loop shape.
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -959,6 +1048,23 @@ struct MatchBatch {
   unsigned int* d_queue = nullptr;
   HeavyTask* d_heavy = nullptr;  // own allocation
   unsigned long long* d_log = nullptr;  // own allocation (root logs)
+  // Budgeted passes (tuned batches): host copies of the layout, the pass
+  // budget and the buffers of the later passes' tasks and abort lists.
+  bool passes = false;
+  uint64_t pass_budget = 0;
+  // Automatic passes (pass_budget 0): the batch's best time in one launch
+  // [0] and in passes [1], negative until measured; a replayed batch runs
+  // the faster.
+  bool pass_auto = false;
+  double pass_ms[2] = {-1.0, -1.0};
+  std::vector<QPlan> h_plans;         // as laid out (task_off order)
+  std::vector<HeavyTask> h_heavy;     // tasks [0, n_heavy) of the first pass
+  std::vector<uint32_t> h_rlen;       // depth-1 slice length per root
+  std::vector<uint32_t> h_rlen_dev;   // what d_rlen holds (lengths or marks)
+  HeavyTask* d_pass = nullptr;        // own allocation
+  size_t pass_cap = 0;
+  unsigned int* d_abort = nullptr;    // own allocation: count, then indices
+  size_t abort_cap = 0;
   uint32_t n_heavy = 0, split_len = 0;
   uint32_t nq = 0, n_tasks = 0;
   int n_queries = 0;
@@ -976,6 +1082,8 @@ void free_batch(void* p) {
   (void)hipFree(b->mem);
   (void)hipFree(b->d_heavy);
   (void)hipFree(b->d_log);
+  (void)hipFree(b->d_pass);
+  (void)hipFree(b->d_abort);
   delete b;
 }
 
@@ -990,6 +1098,7 @@ std::vector<uint32_t> batch_key(const gpm_query* queries, int n_queries, const g
   k.push_back(t.chunk);
   k.push_back((uint32_t)t.use_graph);
   k.push_back((uint32_t)t.set_variant);
+  k.push_back(t.pass_budget);
   if (t.root_model) {
     const uint32_t* w = reinterpret_cast<const uint32_t*>(t.root_model);
     k.insert(k.end(), w, w + kMlpWeights);
@@ -1005,24 +1114,185 @@ std::vector<uint32_t> batch_key(const gpm_query* queries, int n_queries, const g
   return k;
 }
 
-// Enqueues counter reset + enumeration of a prepared batch.
+// Enqueues one enumeration launch over n_tasks tasks, the first n_heavy of
+// them heavy parts at d_heavy and the rest b's light chunks.  zero_counts:
+// clear the counters first (the first launch of a call).  budget > 0: a
+// budgeted count whose stopped tasks are listed in d_abort.
+int launch_tasks(gpm_graph* g, MatchBatch* b, const HeavyTask* d_heavy, uint32_t n_heavy,
+                 uint32_t n_tasks, bool zero_counts, uint64_t budget, unsigned int* d_abort) {
+  hipLaunchKernelGGL(zero_kernel, dim3((b->n_queries + 64) / 64 + 1), dim3(64), 0, g->stream,
+                     b->d_counts, zero_counts ? b->n_queries : 0, b->d_queue, 4);
+  if (budget)
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, g->stream, nullptr, 0, d_abort, 1);
+  auto* kernel = budget ? (b->merge ? &match_kernel<kModeBudget, true>
+                                    : &match_kernel<kModeBudget, false>)
+                 : b->mode == kModeTrial ? &match_kernel<kModeTrial, false>
+                 : b->mode == kModeLog   ? &match_kernel<kModeLog, false>
+                 : b->merge              ? &match_kernel<kModeCount, true>
+                                         : &match_kernel<kModeCount, false>;
+  hipLaunchKernelGGL(kernel, dim3(b->grid), dim3(kBlock), 0, g->stream, b->dg, b->d_plans, b->nq,
+                     n_tasks, b->d_roots, b->d_rlen, d_heavy, n_heavy, b->split_len, b->d_queue,
+                     b->d_counts, b->d_work, budget, d_abort);
+  GPM_HIP(hipGetLastError());
+  return GPM_OK;
+}
+
+// Enqueues counter reset + enumeration of a prepared batch in one launch.
 int launch_batch(gpm_graph* g, MatchBatch* b) {
   if (b->exec) {
     GPM_HIP(hipGraphLaunch(b->exec, g->stream));
     return GPM_OK;
   }
-  hipLaunchKernelGGL(zero_kernel, dim3((b->n_queries + 64) / 64 + 1), dim3(64), 0, g->stream,
-                     b->d_counts, b->n_queries, b->d_queue, 4);
-  auto* kernel = b->mode == kModeTrial ? &match_kernel<kModeTrial, false>
-                 : b->mode == kModeLog  ? &match_kernel<kModeLog, false>
-                 : b->merge             ? &match_kernel<kModeCount, true>
-                                        : &match_kernel<kModeCount, false>;
-  hipLaunchKernelGGL(kernel,
-                     dim3(b->grid),
-                     dim3(kBlock), 0, g->stream, b->dg,
-                     b->d_plans, b->nq, b->n_tasks, b->d_roots, b->d_rlen, b->d_heavy,
-                     b->n_heavy, b->split_len, b->d_queue, b->d_counts, b->d_work);
-  GPM_HIP(hipGetLastError());
+  return launch_tasks(g, b, b->d_heavy, b->n_heavy, b->n_tasks, true, 0, nullptr);
+}
+
+// Last plan with task_off <= t (the host side of find_plan).
+uint32_t host_find_plan(const std::vector<QPlan>& plans, uint32_t t) {
+  uint32_t lo = 0, hi = (uint32_t)plans.size();
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (plans[mid].task_off <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// The parts of a stopped task one level finer, into out: a range of several
+// depth-1 entries is cut into kPassSplit ranges; a single entry gets the
+// first uncut DFS depth d (2 <= d < tail, d <= kCutLevels + 1) cut into
+// kPassSplit ranges.  A task with nothing left to cut goes to `whole`, to
+// be counted without a budget.
+void split_heavy(const MatchBatch& b, const HeavyTask& h, std::vector<HeavyTask>* out,
+                 std::vector<HeavyTask>* whole) {
+  const QPlan& p = b.h_plans[h.plan];
+  if (p.tail <= 1) {
+    whole->push_back(h);
+    return;
+  }
+  const uint32_t hi = std::min(h.hi, b.h_rlen[p.root_off + h.root]);
+  if (hi > h.lo + 1) {
+    const uint32_t step = (hi - h.lo + kPassSplit - 1) / kPassSplit;
+    for (uint32_t a = h.lo; a < hi; a += step) {
+      HeavyTask t = h;
+      t.lo = a;
+      t.hi = std::min(hi, a + step);
+      out->push_back(t);
+    }
+    return;
+  }
+  for (int i = 0; i < kCutLevels; ++i) {
+    if ((uint32_t)i + 2 >= p.tail) break;  // depth i + 2 is not a DFS depth
+    if ((h.cut[i] & 0xFFFFu) != 1u) continue;
+    for (uint32_t k = 0; k < kPassSplit; ++k) {
+      HeavyTask t = h;
+      t.cut[i] = make_cut(k, kPassSplit);
+      out->push_back(t);
+    }
+    return;
+  }
+  whole->push_back(h);
+}
+
+// A stopped light chunk (index lt among the light tasks): one task per root
+// it counts.
+void split_light(const MatchBatch& b, uint32_t lt, std::vector<HeavyTask>* out) {
+  const uint32_t pi = host_find_plan(b.h_plans, lt);
+  const QPlan& p = b.h_plans[pi];
+  const uint32_t r0 = (lt - p.task_off) * p.chunk;
+  const uint32_t r1 = std::min(r0 + p.chunk, p.n_roots);
+  for (uint32_t r = r0; r < r1; ++r) {
+    if (p.tail > 1 && b.h_rlen_dev[p.root_off + r] > b.split_len) continue;  // a heavy task
+    HeavyTask t{pi, r, 0u, p.tail > 1 ? b.h_rlen[p.root_off + r] : 0xFFFFFFFFu,
+                {kNoCut, kNoCut, kNoCut, kNoCut}};
+    out->push_back(t);
+  }
+}
+
+int ensure_pass_buffers(MatchBatch* b, size_t tasks) {
+  if (tasks > b->pass_cap) {
+    (void)hipFree(b->d_pass);
+    b->d_pass = nullptr;
+    b->pass_cap = 0;
+    if (hipMalloc(&b->d_pass, tasks * sizeof(HeavyTask)) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(GPM_ENOMEM, "matching pass tasks (%zu)", tasks);
+    }
+    b->pass_cap = tasks;
+  }
+  if (tasks + 1 > b->abort_cap) {
+    (void)hipFree(b->d_abort);
+    b->d_abort = nullptr;
+    b->abort_cap = 0;
+    if (hipMalloc(&b->d_abort, (tasks + 1) * sizeof(unsigned int)) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(GPM_ENOMEM, "matching abort list (%zu)", tasks + 1);
+    }
+    b->abort_cap = tasks + 1;
+  }
+  return GPM_OK;
+}
+
+// Budgeted passes of a tuned batch.  A search tree can hold most of its
+// nodes below a few depth-2 entries, and one wave walks such a subtree
+// alone, one dependent load chain per node: the launch then lasts as long
+// as that subtree.  So every task runs under a step budget; a task that
+// reaches it stops, adds nothing and is counted again in the next pass, cut
+// one level finer (split_heavy, split_light), until no task stops.  The last
+// pass, and tasks with nothing left to cut, run without a budget.  Every
+// task is counted exactly once, so the counts are those of one launch.
+int run_passes(gpm_graph* g, MatchBatch* b) {
+  std::vector<HeavyTask> cur, next, whole;
+  const HeavyTask* d_tasks = b->d_heavy;
+  uint32_t n_heavy = b->n_heavy, n_tasks = b->n_tasks;
+  int st;
+  for (int pass = 0; n_tasks; ++pass) {
+    const bool last = pass + 1 >= kMaxPasses;
+    const uint64_t budget = last ? 0 : b->pass_budget;
+    if ((st = ensure_pass_buffers(b, std::max<size_t>(n_tasks, 1))) != GPM_OK) return st;
+    if ((st = launch_tasks(g, b, d_tasks, n_heavy, n_tasks, pass == 0, budget, b->d_abort)) !=
+        GPM_OK)
+      return st;
+    if (!budget) break;
+    unsigned int n_ab = 0;
+    GPM_HIP(hipMemcpyAsync(&n_ab, b->d_abort, sizeof(n_ab), hipMemcpyDeviceToHost, g->stream));
+    GPM_HIP(hipStreamSynchronize(g->stream));
+    if (n_ab == 0) break;
+    if (n_ab > n_tasks) return fail(GPM_EHIP, "abort list of %u tasks past %u", n_ab, n_tasks);
+    std::vector<unsigned int> ab(n_ab);
+    GPM_HIP(hipMemcpyAsync(ab.data(), b->d_abort + 1, n_ab * sizeof(unsigned int),
+                           hipMemcpyDeviceToHost, g->stream));
+    GPM_HIP(hipStreamSynchronize(g->stream));
+    next.clear();
+    // Splitting stops (the stopped tasks are counted whole) where the next
+    // pass would hold more than kMaxPassTasks tasks.
+    const bool split = (size_t)n_ab * kPassSplit <= kMaxPassTasks;
+    for (unsigned int t : ab) {
+      if (pass == 0 && t >= n_heavy)
+        split_light(*b, t - n_heavy, split ? &next : &whole);
+      else if (split)
+        split_heavy(*b, pass == 0 ? b->h_heavy[t] : cur[t], &next, &whole);
+      else
+        whole.push_back(pass == 0 ? b->h_heavy[t] : cur[t]);
+    }
+    if (next.size() >= (1ull << 31)) return fail(GPM_EINVAL, "too many matching pass tasks");
+    cur.swap(next);
+    n_heavy = n_tasks = (uint32_t)cur.size();
+    if (n_tasks) {
+      if ((st = ensure_pass_buffers(b, n_tasks)) != GPM_OK) return st;
+      if ((st = h2d(g, b->d_pass, cur.data(), cur.size() * sizeof(HeavyTask))) != GPM_OK)
+        return st;
+    }
+    d_tasks = b->d_pass;
+  }
+  if (!whole.empty()) {
+    const uint32_t n = (uint32_t)whole.size();
+    if ((st = ensure_pass_buffers(b, n)) != GPM_OK) return st;
+    if ((st = h2d(g, b->d_pass, whole.data(), whole.size() * sizeof(HeavyTask))) != GPM_OK)
+      return st;
+    if ((st = launch_tasks(g, b, b->d_pass, n, n, false, 0, nullptr)) != GPM_OK) return st;
+  }
   return GPM_OK;
 }
 
@@ -1041,9 +1311,12 @@ int build_graph(MatchBatch* b) {
   zp.extra = nullptr;
   hipGraphNode_t zn = nullptr, mn = nullptr;
   GPM_HIP(hipGraphAddKernelNode(&zn, b->graph, nullptr, 0, &zp));
+  uint64_t no_budget = 0;
+  unsigned int* no_abort = nullptr;
   void* margs[] = {&b->dg,      &b->d_plans,   &b->nq,      &b->n_tasks,
                    &b->d_roots, &b->d_rlen,    &b->d_heavy, &b->n_heavy,
-                   &b->split_len, &b->d_queue, &b->d_counts, &b->d_work};
+                   &b->split_len, &b->d_queue, &b->d_counts, &b->d_work,
+                   &no_budget,  &no_abort};
   hipKernelNodeParams mp{};
   mp.func = b->merge ? reinterpret_cast<void*>(&match_kernel<kModeCount, true>)
                      : reinterpret_cast<void*>(&match_kernel<kModeCount, false>);
@@ -1380,7 +1653,7 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
         const uint32_t step = (len + np - 1) / np;
         marks[q.root_off + r] = 0xFFFFFFFFu;
         for (uint32_t lo = 0; lo < len; lo += step) {
-          heavy.push_back({li, r, lo, std::min(len, lo + step), 0u, 1u});
+          heavy.push_back({li, r, lo, std::min(len, lo + step), {kNoCut, kNoCut, kNoCut, kNoCut}});
           hlen.push_back(c * (double)(std::min(len, lo + step) - lo) / (double)len);
         }
         continue;
@@ -1388,7 +1661,8 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
       if (len <= kSplitLen) continue;
       for (uint32_t lo = 0; lo < len; lo += kSplitLen) {
         if (trial && !sampled(q.qid, troots[q.root_off + r], lo)) continue;
-        heavy.push_back({li, r, lo, std::min(len, lo + kSplitLen), 0u, 1u});
+        heavy.push_back(
+            {li, r, lo, std::min(len, lo + kSplitLen), {kNoCut, kNoCut, kNoCut, kNoCut}});
         hlen.push_back(len);
       }
     }
@@ -1440,7 +1714,8 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
           marks[q.root_off + r] = 0xFFFFFFFFu;  // no embeddings when len == 0
           for (uint32_t lo = 0; lo < len; lo += part1)
             for (uint32_t k = 0; k < nparts2; ++k) {
-              heavy.push_back({li, r, lo, std::min(len, lo + part1), k, nparts2});
+              heavy.push_back({li, r, lo, std::min(len, lo + part1),
+                               {make_cut(k, nparts2), kNoCut, kNoCut, kNoCut}});
               hlen.push_back(len);
             }
         }
@@ -1478,6 +1753,19 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
       return fail(GPM_ENOMEM, "root work log");
     b->d_work = b->d_log;
     b->mode = kModeLog;
+  }
+  if (tun.autotune && !tun.use_graph && !trial && !log_roots &&
+      tun.pass_budget != kPassBudgetOff && (refined || tun.pass_budget != 0)) {
+    b->passes = true;
+    b->pass_auto = tun.pass_budget == 0;
+    b->pass_budget = tun.pass_budget ? tun.pass_budget : kPassBudget;
+    b->h_plans = laid;
+    b->h_heavy = heavy;
+    b->h_rlen = rlen;
+    if (learned || refined)
+      b->h_rlen_dev.assign(marks.begin(), marks.begin() + root_total);
+    else
+      b->h_rlen_dev = rlen;
   }
   if (tun.use_graph && !trial && !log_roots) return build_graph(b);
   return GPM_OK;
@@ -1697,10 +1985,9 @@ int single_query_batch(gpm_graph* g, const gpm_query* query, int policy, bool lo
 
 }  // namespace
 
-// The root log (gpm_match_root_work, gpm_match_root_features) stopped
-// responding once on the device in its first test and the cause has not been
-// found; both entry points are switched off until it is.
-constexpr bool kRootLogEnabled = false;
+// The root log (gpm_match_root_work, gpm_match_root_features): its stall
+// came from the split task loop described in match_kernel.
+constexpr bool kRootLogEnabled = true;
 
 extern "C" int gpm_match_root_work(gpm_graph* g, const gpm_query* query, int policy,
                                    uint64_t* work) {
@@ -1829,10 +2116,22 @@ extern "C" int gpm_match_batch(gpm_graph* g, const gpm_query* queries, int n_que
     }
     if (b->nq == 0) return GPM_OK;
     int st;
+    // Automatic passes: the first call of a batch runs passes, the second
+    // one launch, later calls the faster so far (a short batch of light
+    // queries, such as the 200 HPRD queries, loses more to the passes'
+    // launches and host round trips than its tail costs: 6.7 against
+    // 2.3 ms).
+    bool passes = b->passes;
+    if (b->pass_auto)
+      passes = b->pass_ms[1] < 0.0 || (b->pass_ms[0] >= 0.0 && b->pass_ms[1] <= b->pass_ms[0]);
     if ((st = timing_begin(g)) != GPM_OK) return st;
-    if ((st = launch_batch(g, b)) != GPM_OK) return st;
+    if ((st = passes ? run_passes(g, b) : launch_batch(g, b)) != GPM_OK) return st;
     timing_mark(g, "match_batch");
     if ((st = timing_end(g)) != GPM_OK) return st;
+    if (b->pass_auto) {
+      double& best = b->pass_ms[passes ? 1 : 0];
+      best = best < 0.0 ? g->last_ms : std::min(best, g->last_ms);
+    }
     std::vector<unsigned long long> cnt(n_queries);
     GPM_HIP(hipMemcpyAsync(cnt.data(), b->d_counts, (size_t)n_queries * 8, hipMemcpyDeviceToHost,
                            g->stream));

@@ -53,7 +53,7 @@ class Query(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("autotune", C.c_int), ("policy", C.c_int),
                 ("chunk", C.c_uint32), ("use_graph", C.c_int),
-                ("root_model", _f32p), ("set_variant", C.c_int), ("reserved", C.c_uint32)]
+                ("root_model", _f32p), ("set_variant", C.c_int), ("pass_budget", C.c_uint32)]
 
 
 _lib = None
@@ -403,14 +403,17 @@ class DeviceGraph:
     def match_batch(self, queries: Sequence, autotune: int = 1,
                     policy: int = SCHED_DEGREE, chunk: int = 0,
                     use_graph: int = 0, model: Optional[np.ndarray] = None,
-                    set_variant: int = 0) -> np.ndarray:
+                    set_variant: int = 0, pass_budget: int = 0) -> np.ndarray:
         """queries: sequence of (n, edges [(u, v), ...], labels [..]); model: packed
         root-cost MLP (4553 float32, see MatchModel) for the LEARNED policy;
-        set_variant: MATCH_SETV_AUTO / _BINARY / _MERGE."""
+        set_variant: MATCH_SETV_AUTO / _BINARY / _MERGE; pass_budget: search
+        steps between a running task's checks for a drained queue, after which
+        it is cut finer and counted again (0: automatic, passes on short
+        batches; PASS_BUDGET_OFF: one launch)."""
         arr, keep = self._queries(queries)
         t = Tuning()
         t.autotune, t.policy, t.chunk, t.use_graph = autotune, policy, chunk, use_graph
-        t.set_variant = set_variant
+        t.set_variant, t.pass_budget = set_variant, pass_budget
         if model is not None:
             self._model = np.ascontiguousarray(model, dtype=np.float32)
             assert self._model.shape == (4553,)
@@ -448,6 +451,7 @@ def tc_edge_noalloc(d_offsets: int, d_adj: int, n: int, m: int, d_count: int,
 
 
 MATCH_SETV_AUTO, MATCH_SETV_BINARY, MATCH_SETV_MERGE = 0, 1, 2
+PASS_BUDGET_OFF = 0xFFFFFFFF  # gpm_tuning.pass_budget: tuned batches in one launch
 SETOP_ISECT_COUNT, SETOP_ISECT, SETOP_DIFF_COUNT, SETOP_DIFF, SETOP_FUSE = 0, 1, 2, 3, 4
 SETV_BINARY, SETV_TWO_PHASE, SETV_MERGE, SETV_HASH, SETV_BITMAP = 0, 1, 2, 3, 4
 

@@ -430,7 +430,18 @@ typedef struct gpm_tuning {
   int set_variant;          /* gpm_match_setv: membership probes of the
                                matcher; AUTO lets the autotuner pick per
                                query (BINARY without autotune)               */
-  uint32_t reserved;
+  uint32_t pass_budget;     /* autotune = 1: budgeted passes.  A running
+                               task checks every pass_budget search steps
+                               (windows + descents) whether the launch's task
+                               queue is drained; then it stops and is counted
+                               again, cut one level finer, in a later launch
+                               of the same call.  0 = automatic: long
+                               batches run in one launch; a short batch runs
+                               passes of 512 steps on its first call, one
+                               launch on its second (an identical call
+                               replays the prepared batch), and the faster of
+                               the two afterwards.  0xFFFFFFFF = always one
+                               launch.  Counts are the same for every value  */
 } gpm_tuning;
 
 /*

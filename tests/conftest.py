@@ -23,6 +23,12 @@ def gpm():
 
 
 @pytest.fixture(scope="session")
+def workloads(gpm):
+    import workloads as W
+    return W
+
+
+@pytest.fixture(scope="session")
 def orc():
     from oracle import oracle
     oracle.lib()

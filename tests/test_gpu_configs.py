@@ -42,12 +42,6 @@ def gold():
         return json.load(f)
 
 
-@pytest.fixture(scope="module")
-def workloads(gpm):
-    import workloads as W
-    return W
-
-
 def test_config2_clique_rmat20(gpm, orc, workloads):
     from cost_model import CostModel
     G = gold()

@@ -184,8 +184,6 @@ def test_learned_model_counts_exact(gpm, hprd, orc):
             assert np.array_equal(d2.match_batch(qs, policy=1, autotune=0, model=model), want)
 
 
-@pytest.mark.skip(reason="stopped responding once on the device (killed after 180 s without "
-                         "output; cause not found by reading the root-log path); not rerun")
 def test_root_work_and_features(gpm):
     g = gpm.rmat_graph(10, 8, 5)
     labels = np.zeros(g.n, dtype=np.uint32)
@@ -202,6 +200,31 @@ def test_root_work_and_features(gpm):
     hop = np.bincount(src, weights=deg[g.adj.astype(np.int64)], minlength=g.n)
     assert np.array_equal(f[cand, 2], hop[cand].astype(np.float32))
     assert np.all(w[~cand] == 0) and np.all(w[cand] > 0)
+
+
+def test_pass_budgets_exact(gpm, orc, workloads):
+    """Tuned batches run in budgeted passes: a task that reaches the step
+    budget stops, adds nothing, and is counted again cut finer (depth-1
+    ranges, then ranges of the depth-2..5 streams).  Dense 9-10-vertex
+    queries of the config-4 generator (deep, narrow search trees) under
+    budgets from 32 steps (most tasks stop and are cut down several levels)
+    to one launch without budget, plain and learned orders, both probe
+    variants: every count equals the oracle's."""
+    g, labels = workloads.labeled_graph(workloads.BATCH_SCALE)
+    qs_all = workloads.batch_queries()
+    dense = [q for q in qs_all if q[0] >= 9 and len(q[1]) >= 22][:10]
+    assert len(dense) >= 6
+    want = orc.ceci_batch(g.offsets, g.adj, labels, dense)
+    assert np.count_nonzero(want) >= 4
+    with gpm.DeviceGraph(g, oriented=False, labels=labels, n_labels=workloads.N_LABELS) as dg:
+        for budget in (32, 500, 0, gpm.PASS_BUDGET_OFF):
+            for kw in (dict(policy=0), dict(policy=1, set_variant=2)):
+                got = dg.match_batch(dense, pass_budget=budget, **kw)
+                print("pass budget", budget, kw, round(dg.last_kernel_ms(), 1), "ms", flush=True)
+                assert np.array_equal(got, want), (budget, kw)
+        # Single queries (the short-batch refinement plus passes).
+        for q, w in list(zip(dense, want))[:3]:
+            assert int(dg.match_batch([q], pass_budget=64)[0]) == int(w)
 
 
 @pytest.mark.parametrize("variant", [1, 2])
