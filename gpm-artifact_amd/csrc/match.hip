@@ -610,7 +610,7 @@ constexpr uint32_t kNoCut = 1u;  // part 0 of 1: the whole stream
 constexpr uint64_t kPassBudget = 512;
 constexpr uint32_t kPassBudgetOff = 0xFFFFFFFFu;  // gpm_tuning.pass_budget: one launch
 constexpr uint32_t kPassSplit = 16;
-constexpr int kMaxPasses = 8;
+constexpr int kMaxPasses = 12;
 constexpr size_t kMaxPassTasks = size_t(1) << 22;
 
 __host__ __device__ constexpr uint32_t make_cut(uint32_t part, uint32_t parts) {
@@ -1195,8 +1195,9 @@ void split_heavy(const MatchBatch& b, const HeavyTask& h, std::vector<HeavyTask>
   whole->push_back(h);
 }
 
-// A stopped light chunk (index lt among the light tasks): one task per root
-// it counts.
+// A stopped light chunk (index lt among the light tasks): per root it
+// counts, kPassSplit ranges of its depth-1 entries (the whole root when its
+// plan has no DFS).
 void split_light(const MatchBatch& b, uint32_t lt, std::vector<HeavyTask>* out) {
   const uint32_t pi = host_find_plan(b.h_plans, lt);
   const QPlan& p = b.h_plans[pi];
@@ -1204,9 +1205,14 @@ void split_light(const MatchBatch& b, uint32_t lt, std::vector<HeavyTask>* out) 
   const uint32_t r1 = std::min(r0 + p.chunk, p.n_roots);
   for (uint32_t r = r0; r < r1; ++r) {
     if (p.tail > 1 && b.h_rlen_dev[p.root_off + r] > b.split_len) continue;  // a heavy task
-    HeavyTask t{pi, r, 0u, p.tail > 1 ? b.h_rlen[p.root_off + r] : 0xFFFFFFFFu,
-                {kNoCut, kNoCut, kNoCut, kNoCut}};
-    out->push_back(t);
+    if (p.tail <= 1) {
+      out->push_back({pi, r, 0u, 0xFFFFFFFFu, {kNoCut, kNoCut, kNoCut, kNoCut}});
+      continue;
+    }
+    const uint32_t len = b.h_rlen[p.root_off + r];
+    const uint32_t step = std::max(1u, (len + kPassSplit - 1) / kPassSplit);
+    for (uint32_t a = 0; a < len; a += step)
+      out->push_back({pi, r, a, std::min(len, a + step), {kNoCut, kNoCut, kNoCut, kNoCut}});
   }
 }
 
@@ -1680,6 +1686,7 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
   constexpr uint64_t kRefineTasksPerWave = 4;
   constexpr uint64_t kRefineBelow = 8;
   const uint64_t have_tasks = (uint64_t)t + heavy.size();
+  const bool short_batch = have_tasks < kRefineBelow * waves;
   bool refined = false;
   if (tun.autotune && !learned && !trial && !log_roots && !tun.chunk &&
       have_tasks < kRefineBelow * waves) {
@@ -1755,7 +1762,7 @@ int build_batch(gpm_graph* g, std::vector<QPlan> plans, int n_counters, const gp
     b->mode = kModeLog;
   }
   if (tun.autotune && !tun.use_graph && !trial && !log_roots &&
-      tun.pass_budget != kPassBudgetOff && (refined || tun.pass_budget != 0)) {
+      tun.pass_budget != kPassBudgetOff && (short_batch || tun.pass_budget != 0)) {
     b->passes = true;
     b->pass_auto = tun.pass_budget == 0;
     b->pass_budget = tun.pass_budget ? tun.pass_budget : kPassBudget;

@@ -44,10 +44,13 @@ def collect(scale, seed, max_rows=8000, log=print):
     feats, works = [], []
     rng = np.random.default_rng(seed)
     with G.DeviceGraph(g, oriented=False, labels=labels, n_labels=8) as dg:
-        for q in queries_for(g, labels, seed):
+        for qi, q in enumerate(queries_for(g, labels, seed)):
             for policy in (G.SCHED_DEGREE, G.SCHED_LEARNED):
+                t0 = time.time()
                 f = dg.match_root_features(q, policy)
                 w = dg.match_root_work(q, policy)
+                print(f"  s{scale} seed {seed} query {qi} ({q[0]} vertices) policy {policy}: "
+                      f"{int(w.sum())} steps, {time.time() - t0:.2f} s", flush=True)
                 rows = np.nonzero(f[:, 3] > 0)[0]  # root candidates (query size > 0)
                 if len(rows) > max_rows:
                     rows = rng.choice(rows, max_rows, replace=False)

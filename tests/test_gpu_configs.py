@@ -21,9 +21,10 @@ tests/test_gpu_tc_clique.py::test_tc_er_workload.  Here:
             five (pyramid-6 there: 6.0e10 embeddings, about 70 s per run) and
             at scale 16.
   config 4  256 seeded connected queries of 4-10 vertices (8 labels) as one
-            batch on labeled R-MAT at the bench's scale (10): oracle parity
-            per query under the plain and learned orders, both probe
-            variants and fixed-chunk graph replay.
+            batch on labeled R-MAT: oracle parity per query at scale 10
+            under the plain and learned orders, both probe variants and
+            fixed-chunk graph replay; at the bench's scale (11) the orders
+            and pass budgets agree.
 """
 import json
 import os
@@ -175,13 +176,30 @@ def test_config3_pyramid_s18_graph_replay(pyramid_s18):
     assert np.array_equal(dg.match_batch(pyr, autotune=0, chunk=8, use_graph=1), plain)
 
 
+def test_config4_batch_bench_scale(gpm, workloads):
+    """Config 4 at the bench's scale (11: 4.5e10 embeddings, one query
+    3.5e10 of them; the oracle would need minutes): the plain and learned
+    orders and a small explicit pass budget (tasks cut down several levels)
+    give identical per-query counts."""
+    g, labels = workloads.labeled_graph(workloads.BATCH_SCALE)
+    qs = workloads.batch_queries()
+    with gpm.DeviceGraph(g, oriented=False, labels=labels, n_labels=8) as dg:
+        ref = dg.match_batch(qs, policy=1)
+        print("config 4 at scale", workloads.BATCH_SCALE, int(ref.sum()), flush=True)
+        assert int(ref.sum()) > 4 * 10 ** 10 and np.count_nonzero(ref) > 200
+        for kw in (dict(policy=0), dict(policy=1, pass_budget=64)):
+            got = dg.match_batch(qs, **kw)
+            print("config 4", kw, dg.last_kernel_ms(), "ms", flush=True)
+            assert np.array_equal(got, ref), kw
+
+
 def test_config4_batch_oracle(gpm, orc, workloads):
     """Config 4's workload (256 seeded connected 4-10-vertex queries, 8
     labels, one batch) at the bench's scale, which the oracle enumerates in
     seconds: every per-query count equals the oracle's under the plain and
     learned orders, both membership-probe variants and fixed-chunk graph
     replay."""
-    g, labels = workloads.labeled_graph(workloads.BATCH_SCALE)
+    g, labels = workloads.labeled_graph(workloads.BATCH_ORACLE_SCALE)
     qs = workloads.batch_queries()
     want = orc.ceci_batch(g.offsets, g.adj, labels, qs)
     print("config 4 oracle total", int(want.sum()), flush=True)

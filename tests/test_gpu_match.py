@@ -208,17 +208,21 @@ def test_pass_budgets_exact(gpm, orc, workloads):
     ranges, then ranges of the depth-2..5 streams).  Dense 9-10-vertex
     queries of the config-4 generator (deep, narrow search trees) under
     budgets from 32 steps (most tasks stop and are cut down several levels)
-    to one launch without budget, plain and learned orders, both probe
-    variants: every count equals the oracle's."""
-    g, labels = workloads.labeled_graph(workloads.BATCH_SCALE)
+    to one launch without budget, plain and learned orders (with and without
+    a root-cost model: its batches keep light root chunks, which stop and
+    are cut into depth-1 ranges), both probe variants: every count equals
+    the oracle's."""
+    g, labels = workloads.labeled_graph(workloads.BATCH_ORACLE_SCALE)
     qs_all = workloads.batch_queries()
     dense = [q for q in qs_all if q[0] >= 9 and len(q[1]) >= 22][:10]
     assert len(dense) >= 6
     want = orc.ceci_batch(g.offsets, g.adj, labels, dense)
     assert np.count_nonzero(want) >= 4
+    model = _models()[-1]  # the trained model when it ships, else a random one
     with gpm.DeviceGraph(g, oriented=False, labels=labels, n_labels=workloads.N_LABELS) as dg:
         for budget in (32, 500, 0, gpm.PASS_BUDGET_OFF):
-            for kw in (dict(policy=0), dict(policy=1, set_variant=2)):
+            for kw in (dict(policy=0), dict(policy=1, set_variant=2),
+                       dict(policy=1, model=model)):
                 got = dg.match_batch(dense, pass_budget=budget, **kw)
                 print("pass budget", budget, kw, round(dg.last_kernel_ms(), 1), "ms", flush=True)
                 assert np.array_equal(got, want), (budget, kw)
