@@ -1243,11 +1243,12 @@ int ensure_pass_buffers(MatchBatch* b, size_t tasks) {
 // Budgeted passes of a tuned batch.  A search tree can hold most of its
 // nodes below a few depth-2 entries, and one wave walks such a subtree
 // alone, one dependent load chain per node: the launch then lasts as long
-// as that subtree.  So every task runs under a step budget; a task that
-// reaches it stops, adds nothing and is counted again in the next pass, cut
-// one level finer (split_heavy, split_light), until no task stops.  The last
-// pass, and tasks with nothing left to cut, run without a budget.  Every
-// task is counted exactly once, so the counts are those of one launch.
+// as that subtree.  So every task checks the task queue every
+// b->pass_budget steps; once the queue is drained (idle waves), it stops,
+// adds nothing and is counted again in the next pass, cut one level finer
+// (split_heavy, split_light), until no task stops.  The last pass, and
+// tasks with nothing left to cut, run without a budget.  Every task is
+// counted exactly once, so the counts are those of one launch.
 int run_passes(gpm_graph* g, MatchBatch* b) {
   std::vector<HeavyTask> cur, next, whole;
   const HeavyTask* d_tasks = b->d_heavy;
@@ -1293,6 +1294,9 @@ int run_passes(gpm_graph* g, MatchBatch* b) {
     d_tasks = b->d_pass;
   }
   if (!whole.empty()) {
+    // The last pass may still read d_pass, which is about to be refilled
+    // (or reallocated).
+    GPM_HIP(hipStreamSynchronize(g->stream));
     const uint32_t n = (uint32_t)whole.size();
     if ((st = ensure_pass_buffers(b, n)) != GPM_OK) return st;
     if ((st = h2d(g, b->d_pass, whole.data(), whole.size() * sizeof(HeavyTask))) != GPM_OK)
