@@ -86,12 +86,16 @@ __device__ __forceinline__ uint32_t select_bit(uint64_t x, uint32_t k) {
 }
 
 // ------------------------------------------------------------ wave tier
+// Wave w takes the chunks of `chunk` roots w, w + waves, ... of the tier's
+// list (ordered by degree or predicted cost, so the interleave evens the
+// waves' loads out).  No task queue: a queue fetch is an atomic on one
+// address, and those serialise (about 10 ns each), which bounded this tier
+// at ~1 M roots per fused pass.
 template <int K, int MODE>
 __global__ __launch_bounds__(kBlock) void clique_wave_kernel(
     const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
     const uint32_t* __restrict__ roots, uint32_t n_roots, uint32_t chunk,
-    unsigned int* __restrict__ queue, unsigned long long* __restrict__ out,
-    unsigned long long* __restrict__ work) {
+    unsigned long long* __restrict__ out, unsigned long long* __restrict__ work) {
   __shared__ uint32_t s_keys[kWavesPerBlock][kWaveTable];
   __shared__ uint32_t s_vals[kWavesPerBlock][kWaveTable];
   __shared__ unsigned long long s_rows[kWavesPerBlock][kWave];
@@ -110,13 +114,11 @@ __global__ __launch_bounds__(kBlock) void clique_wave_kernel(
   uint32_t* svals = s_sv[w];
   constexpr uint32_t hmask = kWaveTable - 1;
   uint64_t acc = 0, acc3 = 0, acc4 = 0;
-  while (true) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(queue, chunk);
-    base = wave_bcast(base, 0);
-    if (base >= n_roots) break;
-    const uint32_t stop = min(base + chunk, n_roots);
-    for (uint32_t r = base; r < stop; ++r) {
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
+  for (uint64_t base = wave * chunk; base < n_roots; base += nwaves * chunk) {
+    const uint32_t stop = (uint32_t)min<uint64_t>(base + chunk, n_roots);
+    for (uint32_t r = (uint32_t)base; r < stop; ++r) {
       const uint32_t u = uniform(roots[r]);
       const uint32_t sb = uniform(off[u]), d = uniform(off[u + 1]) - sb;  // d <= 64 by tiering
       const bool act = lane < d;
@@ -726,7 +728,7 @@ int launch_wave_tier(gpm_graph* g, hipStream_t stream, const RootTiers* t, unsig
   const unsigned blocks =
       (unsigned)std::max<uint64_t>(1, (waves + kWavesPerBlock - 1) / kWavesPerBlock);
   hipLaunchKernelGGL((clique_wave_kernel<K, MODE>), dim3(blocks), dim3(kBlock), 0, stream,
-                     g->toff, g->tadj, t->d_wave, t->n_wave, chunk, q, out, work);
+                     g->toff, g->tadj, t->d_wave, t->n_wave, chunk, out, work);
   GPM_HIP(hipGetLastError());
   if ((MODE == kCount || MODE == kAll) && stream == g->stream) timing_mark(g, "clique_wave");
   return GPM_OK;

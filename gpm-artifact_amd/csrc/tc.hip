@@ -160,32 +160,38 @@ __global__ __launch_bounds__(kBlock) void tc_sum_kernel(
   }
 }
 
-// Wave per root: roots[] lists vertices with 2 <= |N+| <= kTcWaveCap.
+// Wave per root: roots[] lists vertices with 2 <= |N+| <= kTcWaveCap, in
+// chunks of `chunk` roots; wave w takes chunks w, w + waves, ... (the list is
+// ordered by degree, so the interleave evens the waves' loads out).  No task
+// queue and one atomic per block: atomics to one address serialise (about
+// 10 ns each), and with one queue fetch per chunk and one sum per wave they
+// were most of the kernel's time on Erdos-Renyi n = 16384.
 __global__ __launch_bounds__(kBlock) void tc_vertex_wave_kernel(
     const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
     const uint32_t* __restrict__ roots, uint32_t n_roots, uint32_t chunk,
-    unsigned int* __restrict__ queue, unsigned long long* out) {
+    unsigned long long* out) {
   constexpr int kWavesPerBlock = kBlock / kWave;
   __shared__ uint32_t s_keys[kWavesPerBlock][kTcWaveTable];
   __shared__ uint32_t s_pre[kWavesPerBlock][kWave];
   __shared__ uint32_t s_beg[kWavesPerBlock][kWave];
+  __shared__ unsigned long long s_sum[kWavesPerBlock];
   const uint32_t lane = lane_id();
   const uint32_t w = threadIdx.x / kWave;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
   uint32_t* keys = s_keys[w];
   uint32_t* pre = s_pre[w];
   uint32_t* beg = s_beg[w];
-  constexpr uint32_t mask = kTcWaveTable - 1;
   uint64_t cnt = 0;
-  while (true) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(queue, chunk);
-    base = wave_bcast(base, 0);
-    if (base >= n_roots) break;
-    const uint32_t stop = min(base + chunk, n_roots);
-    for (uint32_t r = base; r < stop; ++r) {
+  for (uint64_t base = wave * chunk; base < n_roots; base += nwaves * chunk) {
+    const uint32_t stop = (uint32_t)min<uint64_t>(base + chunk, n_roots);
+    for (uint32_t r = (uint32_t)base; r < stop; ++r) {
       const uint32_t u = uniform(roots[r]);
       const uint32_t sb = uniform(off[u]), d = uniform(off[u + 1]) - sb;
-      for (uint32_t i = lane; i < kTcWaveTable; i += kWave) keys[i] = kEmpty;
+      // Open-addressing table of at least 2d slots (d <= kTcWaveCap).
+      const uint32_t size = d <= 32 ? 64u : d <= 64 ? 128u : d <= 128 ? 256u : kTcWaveTable;
+      const uint32_t mask = size - 1;
+      for (uint32_t i = lane; i < size; i += kWave) keys[i] = kEmpty;
       __builtin_amdgcn_wave_barrier();
       for (uint32_t i = lane; i < d; i += kWave) {
         const uint32_t key = adj[sb + i];
@@ -219,7 +225,13 @@ __global__ __launch_bounds__(kBlock) void tc_vertex_wave_kernel(
     }
   }
   cnt = wave_sum_u64(cnt);
-  if (lane == 0 && cnt) atomicAdd(out, (unsigned long long)cnt);
+  if (lane == 0) s_sum[w] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long total = 0;
+    for (int i = 0; i < kWavesPerBlock; ++i) total += s_sum[i];
+    if (total) atomicAdd(out, total);
+  }
 }
 
 // Block-wide inclusive scan of one value per thread (kBlock threads).
@@ -391,7 +403,7 @@ int enqueue_tc(gpm_graph* g, int variant, bool mark) {
     const uint64_t waves = std::min<uint64_t>((t->n_wave + chunk - 1) / chunk, (uint64_t)cus * 32);
     const unsigned blocks = (unsigned)std::max<uint64_t>(1, (waves * kWave + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(tc_vertex_wave_kernel, dim3(blocks), dim3(kBlock), 0, g->stream, g->off,
-                       g->adj, t->d_wave, t->n_wave, chunk, q, g->counters);
+                       g->adj, t->d_wave, t->n_wave, chunk, g->counters);
     GPM_HIP(hipGetLastError());
     if (mark) timing_mark(g, "tc_vertex_wave");
   }

@@ -163,7 +163,9 @@ int get_tiers(gpm_graph* g, const TierRequest& req, const RootTiers** out) {
   t.n_block2 = (uint32_t)block2.size();
   t.max_block2_deg = maxb2;
   t.max_big_deg = maxg;
-  t.wave_chunk = req.chunk ? req.chunk : 8;
+  // One root per chunk: on R-MAT s20 the 5-clique wave tier took 1.20 ms
+  // interleaved by single roots, 1.47 ms by 4, 1.91 ms by 8.
+  t.wave_chunk = req.chunk ? req.chunk : 1;
   if (req.rank)
     for (auto* tier : {&wave, &block, &block2, &big})
       for (uint32_t& v : *tier) v = req.rank[v];

@@ -34,7 +34,7 @@ struct RootTiers {
   uint32_t* d_big_parts = nullptr;
   uint32_t n_split = 0;            // roots split into more than one part
   unsigned int* queues = nullptr;  // 4 work-queue counters
-  uint32_t wave_chunk = 8;         // roots per wave fetch
+  uint32_t wave_chunk = 1;         // roots per chunk of the wave tier's interleave
 };
 
 // What a cached RootTiers was built for (see key_of in tiers.hip).

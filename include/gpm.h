@@ -225,7 +225,9 @@ enum gpm_sched_policy {
 
 typedef struct gpm_sched {
   int policy;              /* gpm_sched_policy                               */
-  uint32_t chunk;          /* roots per work-queue fetch, 0 = automatic       */
+  uint32_t chunk;          /* k-clique wave tier: roots per chunk of its
+                              static interleave (wave w takes chunks w,
+                              w + waves, ...), 0 = automatic (1)             */
   uint32_t root_begin;     /* root range [root_begin, root_end) (vertex ids); */
   uint32_t root_end;       /*   root_end == 0 means "all vertices"            */
   const float* root_cost;  /* LEARNED: predicted cost per vertex (n floats)   */
