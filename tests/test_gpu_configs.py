@@ -10,8 +10,8 @@ tests/test_gpu_tc_clique.py::test_tc_er_workload.  Here:
             present; k=4 and k=5 equal the oracle on a seeded uniform random
             sample of roots; the learned schedule gives the same counts.
   config 5  R-MAT scale 22 (seed 20240522): 5-clique learned == plain ==
-            degree-split; triangles and 4-cliques equal the oracle's full
-            counts; the induced Motif4 counts satisfy the non-induced
+            degree-split == the oracle's full count (15 548 s on 6 threads);
+            triangles and 4-cliques equal the oracle's full counts; the induced Motif4 counts satisfy the non-induced
             identities checked against independently computed totals, and at
             scale 16 (the largest the oracle finishes in minutes) they equal
             the oracle's counts.
