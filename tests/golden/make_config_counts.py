@@ -4,7 +4,7 @@ itself pinned by closed forms, networkx and brute-force census in
 tests/test_oracle.py).  The GPU tests compare the HIP engine with these at the
 configurations' full sizes, where the oracle takes minutes to hours.
 
-    python tests/golden/make_config_counts.py [--threads T] [--skip-k5-s22]
+    python tests/golden/make_config_counts.py [--threads T] [--only NAME,...]
 
 Each entry records the graph (generator, scale, seed), what was counted and
 the oracle's wall time on this host.
@@ -88,6 +88,27 @@ def main():
                     "motif4_induced": [int(x) for x in c], "oracle_seconds": t}
         return f
 
+    def named_entry(scale):
+        def f():
+            import workloads as W
+            g, labels = W.labeled_graph(scale)
+            named = dict(W.named_queries())
+            names = ["triangle", "4-cycle", "diamond", "three-star", "pyramid-6"]
+            c, t = timed(O.ceci_batch, g.offsets, g.adj, labels, [named[q] for q in names])
+            return {"graph": f"labeled R-MAT scale {scale} (workloads.labeled_graph, 8 labels)",
+                    "queries": names, "counts": [int(x) for x in c], "oracle_seconds": t}
+        return f
+
+    def batch_entry(scale):
+        def f():
+            import workloads as W
+            g, labels = W.labeled_graph(scale)
+            c, t = timed(O.ceci_batch, g.offsets, g.adj, labels, W.batch_queries())
+            return {"graph": f"labeled R-MAT scale {scale} (workloads.labeled_graph, 8 labels)",
+                    "queries": "workloads.batch_queries()", "counts": [int(x) for x in c],
+                    "oracle_seconds": t}
+        return f
+
     run("rmat20_k3", clique_entry(20, 20240520, 3))
     run("rmat20_k4", clique_entry(20, 20240520, 4))
     run("rmat16_s20240522_motif4", motif_entry(16, 20240522))
@@ -95,6 +116,8 @@ def main():
     run("rmat22_k4", clique_entry(22, 20240522, 4))
     run("rmat20_k5", clique_entry(20, 20240520, 5))
     run("rmat22_k5", clique_entry(22, 20240522, 5))
+    run("labeled18_named", named_entry(18))
+    run("labeled11_batch256", batch_entry(11))
 
 
 if __name__ == "__main__":

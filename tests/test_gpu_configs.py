@@ -142,6 +142,9 @@ def test_config3_named_queries_s18(gpm, workloads):
     with gpm.DeviceGraph(g, oriented=False, labels=labels, n_labels=8) as dg:
         plain = dg.match_batch(four, policy=0)
         print("config 3 s18 counts", plain.tolist(), flush=True)
+        G = gold()
+        if "labeled18_named" in G:  # the oracle's full counts, when computed
+            assert plain.tolist() == G["labeled18_named"]["counts"][:4]
         assert np.array_equal(dg.match_batch(four, policy=1), plain)
         assert np.array_equal(dg.match_batch(four, autotune=0, chunk=8, use_graph=1), plain)
     # pyramid-6 grows ~10x per two scales (3.1e9 embeddings at scale 16).
@@ -162,6 +165,9 @@ def pyramid_s18(gpm, workloads):
     dg = gpm.DeviceGraph(g, oriented=False, labels=labels, n_labels=8)
     plain = dg.match_batch(pyr, policy=0)
     print("config 3 pyramid-6 s18", int(plain[0]), dg.last_kernel_ms(), "ms", flush=True)
+    G = gold()
+    if "labeled18_named" in G:  # the oracle's full count, when computed
+        assert int(plain[0]) == G["labeled18_named"]["counts"][4]
     yield dg, pyr, plain
     dg.close()
 
@@ -186,6 +192,10 @@ def test_config4_batch_bench_scale(gpm, workloads):
     with gpm.DeviceGraph(g, oriented=False, labels=labels, n_labels=8) as dg:
         ref = dg.match_batch(qs, policy=1)
         print("config 4 at scale", workloads.BATCH_SCALE, int(ref.sum()), flush=True)
+        G = gold()
+        key = f"labeled{workloads.BATCH_SCALE}_batch256"
+        if key in G:  # the oracle's per-query counts, when computed
+            assert ref.tolist() == G[key]["counts"]
         assert int(ref.sum()) > 4 * 10 ** 10 and np.count_nonzero(ref) > 200
         for kw in (dict(policy=0), dict(policy=1, pass_budget=64)):
             got = dg.match_batch(qs, **kw)
