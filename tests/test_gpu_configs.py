@@ -18,7 +18,7 @@ tests/test_gpu_tc_clique.py::test_tc_er_workload.  Here:
   config 3  the five named queries (triangle, 4-cycle, diamond, three-star,
             pyramid-6) on labeled R-MAT: oracle parity at scale 12; plain,
             learned and fixed-chunk graph replay agree at scale 18 for all
-            five (pyramid-6 there: 6.0e10 embeddings, about 70 s per run) and
+            five and equal the oracle's full counts there (pyramid-6 there: 6.0e10 embeddings, about 70 s per run) and
             at scale 16.
   config 4  256 seeded connected queries of 4-10 vertices (8 labels) as one
             batch on labeled R-MAT: oracle parity per query at scale 10
