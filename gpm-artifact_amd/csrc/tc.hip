@@ -4,11 +4,14 @@
 //                per oriented edge (u, v); each lane takes elements of the
 //                shorter list and binary-searches the longer one, or, when
 //                the longer list has at most 2G entries, the group compares
-//                the two lists pairwise by lane rotation.  G follows the mean
-//                out-degree so that low-degree graphs keep all 64 lanes busy.
+//                the two lists' chunks pairwise by XOR lane exchanges
+//                (ds_swizzle).  G follows the mean out-degree so that
+//                low-degree graphs keep all 64 lanes busy.  Blocks store
+//                partial sums that a one-block kernel adds up.
 //  GPM_TC_VERTEX (v_centric, TCSolver.cuh:64-113, tc_hi_warp_vertex.cuh):
-//                one wave per root u with |N+(u)| <= 256, one 256-thread
-//                block per larger root.  N+(u) goes into an LDS hash table;
+//                one wave per root u with |N+(u)| <= 256 (roots interleaved
+//                statically over the waves), one 256-thread block per larger
+//                root.  N+(u) goes into an LDS hash table;
 //                the lists N+(v), v in N+(u), are walked as one flattened
 //                range so that every lane reads one entry per step
 //                (coalesced, no idle lanes on short lists).  Roots above the
