@@ -33,7 +33,7 @@ struct RootTiers {
   uint32_t* d_block2_parts = nullptr;
   uint32_t* d_big_parts = nullptr;
   uint32_t n_split = 0;            // roots split into more than one part
-  unsigned int* queues = nullptr;  // 4 work-queue counters
+  unsigned int* queues = nullptr;  // work-queue counters of the block tiers (4)
   uint32_t wave_chunk = 1;         // roots per chunk of the wave tier's interleave
 };
 
