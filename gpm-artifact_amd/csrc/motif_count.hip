@@ -46,14 +46,16 @@ constexpr int kBlock = 256;
 // shapes measured at the launch).
 constexpr int kCycBlock = 256;
 constexpr int kCycBlocksPerCu = 1;
-constexpr uint32_t kWedgesInFlight = 4;
+constexpr uint32_t kWedgesInFlight = 8;
 // Counters of the wedge endpoints ranked just below the root, [u - kCycBand,
 // u), live in LDS rather than in the block's HBM array.  On R-MAT scale 22,
-// 43.6 % of the wedges of the top 1 % of roots end in that band (hubs link
-// to hubs), and the walk takes 1.08 s instead of 1.39 s with every counter
-// in HBM.  32768 words are 128 KB of the CU's 160 KB of LDS; the kernel runs
-// one block per CU.
-constexpr uint32_t kCycBand = 32768;
+// 32768 ranks below the root hold 43.6 % of the wedges of the top 1 % of
+// roots (hubs link to hubs).  The walk took 1.39 s with every counter in
+// HBM; with the band (words, wedges in flight): 32768 x 4 1.08 s,
+// 32768 x 8 1.07 s, 40000 x 4 1.05 s, 40000 x 8 1.04 s.  40000 words with
+// the walk's other arrays use 158 KB of the CU's 160 KB of LDS; the kernel
+// runs one block per CU.
+constexpr uint32_t kCycBand = 40000;
 
 // Blocks of 256 threads for `threads` threads, at most 32 per CU (the
 // kernels stride over the rest).
@@ -197,8 +199,9 @@ __device__ __forceinline__ void cycle_walk(const uint32_t* __restrict__ off,
     // kWedgesInFlight wedges per thread per step: their endpoint loads and
     // counter atomics are independent, so their latencies overlap (the walk
     // runs few waves per CU: one block per CU holds the counters).  On R-MAT
-    // scale 22 (device-scope fences per root): 1 in flight 1.96 s, 4 1.85 s,
-    // 8 1.88 s; with workgroup-scope atomics and fences 1.39 s.  Aggregating a
+    // scale 22, before the LDS band (device-scope fences per root): 1 in
+    // flight 1.96 s, 4 1.85 s, 8 1.88 s; with workgroup-scope atomics and
+    // fences 1.39 s.  With the band, 8 in flight beat 4.  Aggregating a
     // root's increments in an LDS hash table first (4.5 wedges per distinct
     // endpoint on the heaviest roots) measured 3.01 s: the table's inserts
     // and the barriers of its flushes cost more than the atomics saved.
